@@ -1,0 +1,190 @@
+"""Benchmark: site updates/s of batched parallel tempering at L=32 on one MI355X.
+
+Workload (one step = one PT cycle over the whole batch, inputs resident in HBM):
+  L = H = 32 (32768 Ising spins), 16 replicas on a linear ladder T = 8.0-11.5
+  x 64 disorder configs (PHI = 2, XI = 3, seeds IRINIT + iconf - 1), STR-X start,
+  PTSWAPEVER = 5 sweeps + one swap pass + the measurement accumulation
+  (moments, histogram, local nematic sums) -- the Pt_measurement_loop cycle.
+  Site updates per step = 32768 * 16 * 64 * 5 = 167,772,160.
+
+Printed as one JSON line: the FP32 batched form's rate (value), the plain FP64
+form's rate on the same workload, the sweep kernel's roofline figures and the
+reference's CPU rate on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "j1j2-public_amd"))
+
+L, NREP, NCONF, PTSWAPEVER = 32, 16, 64, 5
+HBM_PEAK = 8.0e12                  # MI355X HBM3E, bytes/s
+SEED = 20261203
+
+
+def build_batch(J, precision, flags=0):
+    g = J.make_params(L, L, nreplica=NREP, nconf=NCONF, iconf0=1, ptswapever=PTSWAPEVER,
+                      precision=precision, flags=flags | J.FLAG_WRITELOCNEM, tmin=8.0, tmax=11.5,
+                      seed=SEED)
+    eng = J.Engine([g])
+    stripes = J.initial_spins(L, L, J.START_STR, 0, 0)
+    import numpy as np
+    spins = np.tile(stripes, NREP)
+    for c in range(NCONF):
+        eng.upload_disorder(c, J.make_disorder(L, L, 3.0, 1 + c, True, 2.0))
+        eng.set_spins(c, spins)
+    return eng
+
+
+def time_form(J, precision, steps, warmup, kernel_timing):
+    eng = build_batch(J, precision)
+    eng.run(J.PHASE_EQUIL, warmup)
+    eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
+    eng.set_timing(kernel_timing)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    sweep_ms, launches = eng.timing(0)
+    path, bytes_per = eng.sweep_path()
+    eng.set_timing(False)
+    fetch = eng.fetch(0)
+    eng.close()
+    su = float(L * L * L) * NREP * NCONF * PTSWAPEVER * steps
+    return dict(seconds=dt, su=su, rate=su / dt, sweep_ms=sweep_ms, launches=launches, path=path,
+                bytes_per=bytes_per, e_slot0=float(fetch["basefields"][9][0]))
+
+
+def mismatch_rate(J, cycles):
+    eng = build_batch(J, J.FP32_BATCHED, J.FLAG_SHADOW_FP64)
+    eng.run(J.PHASE_EQUIL, cycles)
+    tot = mis = 0
+    for c in range(NCONF):
+        a, b = eng.mismatch(c)
+        tot += a
+        mis += b
+    eng.close()
+    return mis / max(tot, 1), tot
+
+
+def traffic_from_profiles():
+    """HBM bytes per sweep-kernel launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(REPO, "profiles", "sweep_resident_l32_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline():
+    """The unmodified reference (oracle/_ref/j1j2_cpu_l32: L=32, NREPLICA=16,
+    PTSWAPEVER=5, NEQ=100 cycles, one disorder config per MPI rank) on this
+    host's cores; su/s from its own 'timing PT_EQUIL' lines."""
+    exe = os.path.join(REPO, "oracle", "_ref", "j1j2_cpu_l32")
+    mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+    if os.path.exists(exe) and os.path.exists(mpiexec):
+        ranks = min(16, os.cpu_count() or 1)
+        tmp = tempfile.mkdtemp(prefix="j1j2cpu")
+        env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib:" + os.environ.get("LD_LIBRARY_PATH", ""))
+        try:
+            out = subprocess.run([mpiexec, "-n", str(ranks), exe], cwd=tmp, env=env, timeout=300,
+                                 capture_output=True, text=True)
+            secs = {}
+            for m in re.finditer(r"timing PT_EQUIL rank\s*(\d+) iconf\s*(\d+) seconds\s*([0-9.]+)", out.stdout):
+                secs.setdefault(int(m.group(1)), []).append(float(m.group(3)))
+            if secs:
+                per_cfg = float(L * L * L) * NREP * 100 * PTSWAPEVER
+                # ranks run concurrently: each rank's rate over its first config
+                rate = sum(per_cfg / v[0] for v in secs.values() if v[0] > 0)
+                return dict(value=rate, unit="site updates/s", cores=len(secs), kind="reference",
+                            sample=f"J1J2_6tv.f90 built from source (amdflang -O2, MPICH), L=32, "
+                                   f"NREPLICA=16, 100 PT cycles x 5 sweeps, one config per rank, "
+                                   f"{len(secs)} ranks; rate from its PT_EQUIL timing lines")
+        except (subprocess.SubprocessError, OSError):
+            pass
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+    # fall back to the oracle's restatement, single thread
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import oracle as O
+    import j1j2_engine as J
+    p = J.make_disorder(L, L, 3.0, 1, True, 2.0)
+    s0 = np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), NREP)
+    ora = O.OraclePT(L, L, p, O.ladder(0, NREP, 8.0, 11.5), s0, precision=1)
+    ora.begin_phase()
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 10.0:
+        ora.cycle(False)
+        n += 1
+    dt = time.perf_counter() - t0
+    return dict(value=float(L ** 3) * NREP * PTSWAPEVER * n / dt, unit="site updates/s", cores=1,
+                kind="port", sample=f"oracle FP32 rule, one config, {n} PT cycles, single thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--fp64-steps", type=int, default=0, help="cycles for the FP64 form (default steps/10)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; the path would shard disorder configs "
+              "across GPUs with no exchange during the run (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import j1j2_engine as J
+
+    fp32 = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=True)
+    res = {
+        "metric": "site updates/s (all replicas) at L=32, FP32 vs FP64 form; % HBM peak",
+        "value": fp32["rate"],
+        "unit": "site updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * fp32["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched PT, L=32, 16 replicas (T=8.0-11.5) x 64 disorder configs "
+                               "(PHI=2, XI=3), 5 sweeps + swap + measurement per step",
+                   "L": L, "H": L, "nreplica": NREP, "nconf": NCONF, "ptswapever": PTSWAPEVER,
+                   "sweep_path": J.PATH_NAMES[fp32["path"]]},
+    }
+    su_launch = fp32["su"] / max(fp32["launches"], 1)
+    avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)
+    achieved = su_launch * fp32["bytes_per"] / avg_s
+    traffic = traffic_from_profiles()
+    res["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK, "traffic": traffic,
+                       "kernel": "k_sweep_resident", "avg_launch_ms": avg_s * 1e3,
+                       "site_updates_per_launch": su_launch,
+                       "algorithmic_bytes_per_update": fp32["bytes_per"]}
+    if not args.no_fp64:
+        n64 = args.fp64_steps or max(args.steps // 10, 5)
+        fp64 = time_form(J, J.FP64_PLAIN, n64, max(args.warmup // 10, 2), kernel_timing=False)
+        res["fp64_plain"] = {"value": fp64["rate"], "steps": n64, "ms_per_step": 1e3 * fp64["seconds"] / n64,
+                             "fp32_over_fp64": fp32["rate"] / fp64["rate"]}
+        rate, n = mismatch_rate(J, 4)
+        res["flip_mismatch_rate"] = {"fp32_vs_fp64": rate, "proposals": n}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+// device_common.h -- device-side building blocks of the J1-J2 engine (gfx950).
+//
+// The arithmetic here defines the engine's update rule; a CPU restatement of
+// the same rule is the test oracle.  Every floating-point expression that feeds
+// a flip decision is written as explicit operations (the file is compiled with
+// -ffp-contract=off), so CPU and GPU agree bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace j1j2 {
+
+// ---------------------------------------------------------------------------
+// Descriptors shared by host and device
+// ---------------------------------------------------------------------------
+struct LatDesc {          // one replica lattice
+    int32_t L, H, N;      // N = L*L*H
+    int32_t conf;         // global config index
+    int32_t rep;          // replica index within its config
+    int32_t slot_base;    // index of the config's slot 0 in per-slot arrays
+    int64_t spin_off;     // byte offset of this lattice's spins
+    int64_t site_off;     // offset (in sites) of the config's per-site planes
+};
+
+struct ConfDesc {         // one disorder config
+    int32_t L, H, N, nrep;
+    int32_t lat0;         // first lattice index of the config
+    int32_t slot_base;    // per-slot arrays offset (slot-major blocks of nrep)
+    int32_t pair_base;    // per-pair arrays offset (nrep-1 entries)
+    int32_t iconf;        // Philox key word 1
+    int64_t site_off;     // per-site planes offset
+    int64_t hist_off;     // histogram offset (nrep*bins)
+    int64_t series_off;   // LOCAL series offset (nrep*5*cap)
+    int64_t flip_off;     // flip-count offset (nrep*N)
+};
+
+struct DeviceClock {      // advanced by a one-thread kernel at the end of each cycle
+    uint64_t cycle;       // cycles completed since the engine was created
+    uint64_t phase_start; // cycle at which the current phase began
+};
+
+// Per-lattice integer observables written after every cycle.
+enum { OBS_M = 0, OBS_SX, OBS_SY, OBS_BX, OBS_BY, OBS_BD, OBS_BZ, OBS_EQ, OBS_N };
+
+// Per-slot moment sums, in the accumulation order of Pt_measurement_loop
+// (J1J2_6tv.f90:1493-1512).
+enum { NSUMS = 20 };
+
+// Internal spin layout: row r = z*L + y holds L bytes, the even-x half first
+// ([x=0,2,..,L-2]) then the odd-x half; a byte is b = (s+1)/2.
+__host__ __device__ inline int64_t internal_index(int32_t L, int32_t x, int32_t y, int32_t z)
+{
+    return ((int64_t)z * L + y) * L + (x & 1) * (L >> 1) + (x >> 1);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Round i uses key + i*(W0, W1).
+// The 32x32->64 products map onto v_mad_u64_u32.
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                             uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// ---------------------------------------------------------------------------
+// Acceptance threshold of the FP32 rule: 2^24 * 2^y for y >= -64.
+// y = n + f, n = rint(y), |f| <= 1/2, 2^f by a degree-6 near-minimax
+// polynomial (relative error < 1.6e-8) with coefficients pre-scaled by 2^24.
+// A proposal with uphill energy is accepted iff float(r >> 8) < threshold.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float accept_threshold_f32(float y)
+{
+    y = fmaxf(y, -64.0f);
+    float n = rintf(y);
+    float f = y - n;
+    float p = 0x1.41d332p+11f;
+    p = __builtin_fmaf(p, f, 0x1.5f456ap+14f);
+    p = __builtin_fmaf(p, f, 0x1.3b2dbcp+17f);
+    p = __builtin_fmaf(p, f, 0x1.c6aed4p+19f);
+    p = __builtin_fmaf(p, f, 0x1.ebfbdap+21f);
+    p = __builtin_fmaf(p, f, 0x1.62e430p+23f);
+    p = __builtin_fmaf(p, f, 0x1.000000p+24f);
+    return ldexpf(p, (int)n);
+}
+
+// ---------------------------------------------------------------------------
+// e^x in FP64: Cody & Waite reduction x = k ln2 + r (ln2 split so k*ln2_hi is
+// exact), Taylor series of e^r to degree 13 in Horner form, ldexp.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double exp64(double x)
+{
+    if (x < -700.0) return 0.0;
+    if (x > 709.0) return __builtin_huge_val();
+    const double LN2_HI = 0x1.62e42feep-1, LN2_LO = 0x1.a39ef35793c76p-33,
+                 INV_LN2 = 0x1.71547652b82fep+0;
+    double k = __builtin_rint(x * INV_LN2);
+    double r = __builtin_fma(-k, LN2_HI, x);
+    r = __builtin_fma(-k, LN2_LO, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+    p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+    p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+    p = __builtin_fma(p, r, 0x1.0000000000000p-1);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, (int)k);
+}
+
+}  // namespace j1j2

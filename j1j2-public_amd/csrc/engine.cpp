@@ -1,0 +1,962 @@
+// engine.cpp -- host side of the J1-J2 MI355X engine: parameter validation,
+// device buffers, per-cycle launch schedule (eager or replayed hipGraph),
+// spin-layout conversion and the exported C-ABI (include/j1j2_engine.h).
+#include "j1j2_engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_model.h"
+#include "kernels.h"
+
+using namespace j1j2;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? J1J2_ENOMEM : J1J2_EDEVICE,               \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+constexpr double LOG2E = 0x1.71547652b82fep+0;
+
+struct Group {
+    j1j2_params p;
+    int32_t conf0, lat0, nlat, N;
+    bool resident;
+};
+
+struct Timer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> live, pool;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Group> groups;
+    j1j2_params common{};
+    std::vector<ConfDesc> confs;
+    std::vector<ConfConst> cconst;
+    std::vector<LatDesc> lats;
+    std::vector<std::vector<double>> temps;   // per config
+    int64_t nsites = 0, nslots = 0, npairs = 0, nhist = 0, nseries = 0, nflips = 0, spin_bytes = 0;
+    int32_t max_n = 0;
+    int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global
+    bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
+    uint64_t cycles_done = 0;
+    bool observed = false;                     // observables valid for the current state
+    std::vector<void *> allocs;
+    DeviceState st{};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    Timer timers[4];
+
+    ~Engine()
+    {
+        for (auto &g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto &t : timers) {
+            for (auto &e : t.live) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+            for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        }
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    int alloc(T *&ptr, int64_t n)
+    {
+        void *p = nullptr;
+        const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+        HIPCHK(hipMalloc(&p, bytes));
+        allocs.push_back(p);
+        HIPCHK(hipMemsetAsync(p, 0, bytes, stream));
+        ptr = static_cast<T *>(p);
+        return 0;
+    }
+};
+
+int validate_group(const j1j2_params &p)
+{
+    // Validate_run_options (J1J2_6tv.f90:841-925) plus the engine's own limits.
+    if (p.L < 2 || p.H < 2 || (p.L & 1) || (p.H & 1))
+        return fail(J1J2_EINVAL, "L and H must be even and >= 2 (four-colour sweep)");
+    if ((int64_t)p.L * p.L * p.H > (1 << 22)) return fail(J1J2_EINVAL, "L*L*H must not exceed 2^22");
+    if (p.sampler != J1J2_SAMPLER_PT && p.sampler != J1J2_SAMPLER_LOCAL)
+        return fail(J1J2_EINVAL, "SAMPLERMODE must be LOCAL or PT");
+    if (p.precision != J1J2_FP64_PLAIN && p.precision != J1J2_FP32_BATCHED)
+        return fail(J1J2_EINVAL, "precision must be FP64_PLAIN or FP32_BATCHED");
+    if (p.ladder != J1J2_LADDER_LINEAR && p.ladder != J1J2_LADDER_SPLIT && p.ladder != J1J2_LADDER_LOCAL)
+        return fail(J1J2_EINVAL, "PTLADDERMODE must be LINEAR or SPLIT (or LOCAL spacing)");
+    if (p.nreplica < 1 || p.nreplica > 256) return fail(J1J2_EINVAL, "NREPLICA must be in [1, 256]");
+    if (p.nconf < 1) return fail(J1J2_EINVAL, "NCONF must be >= 1");
+    if (p.bins < 1) return fail(J1J2_EINVAL, "BINS must be >= 1");
+    if (p.ptswapever <= 0) return fail(J1J2_EINVAL, "PTSWAPEVER must be positive");
+    if (p.sampler == J1J2_SAMPLER_LOCAL && p.ptswapever != 1)
+        return fail(J1J2_EINVAL, "LOCAL sampler measures every sweep: PTSWAPEVER must be 1");
+    if (p.ptmeasindex < 1 || p.ptmeasindex > p.nreplica) return fail(J1J2_EINVAL, "PTMEASINDEX out of range");
+    if (p.neq < 0 || p.nmess < 0) return fail(J1J2_EINVAL, "NEQ and NMESS must be >= 0");
+    if (p.tmax < p.tmin) return fail(J1J2_EINVAL, "TMAX must be >= TMIN");
+    if (p.ladder == J1J2_LADDER_LOCAL && !(p.dt > 0.0)) return fail(J1J2_EINVAL, "DT must be positive");
+    if (p.ladder == J1J2_LADDER_SPLIT) {
+        if (p.nreplica < 3) return fail(J1J2_EINVAL, "SPLIT ladder requires at least 3 temperatures");
+        if (p.tladdermid <= p.tmin || p.tladdermid >= p.tmax)
+            return fail(J1J2_EINVAL, "TLADDERMID must lie strictly inside (TMIN,TMAX)");
+        if (p.tladderfrac <= 0.0 || p.tladderfrac >= 1.0)
+            return fail(J1J2_EINVAL, "TLADDERFRAC must lie strictly inside (0,1)");
+    }
+    if (!(p.tmin > 0.0)) return fail(J1J2_EINVAL, "temperatures must be positive");
+    return 0;
+}
+
+int ref_site(int L, int x, int y, int z) { return x + L * y + L * L * z; }
+
+// host-side FP32 constants, fixed-point disorder and FP64 field of one config,
+// written in the internal site order
+void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p, std::vector<float> kd[5],
+                       std::vector<double> &p64, std::vector<int64_t> &q)
+{
+    const int L = cd.L, H = cd.H, N = cd.N;
+    std::vector<float> Kx(N), Ky(N);
+    for (int z = 0; z < H; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int i = ref_site(L, x, y, z);
+                Kx[i] = (float)((cc.J1h + p[i]) + p[ref_site(L, (x + 1) % L, y, z)]);
+                Ky[i] = (float)((cc.J1v - p[i]) - p[ref_site(L, x, (y + 1) % L, z)]);
+            }
+    for (int P = 0; P < 5; ++P) kd[P].assign(N, 0.0f);
+    p64.assign(N, 0.0);
+    q.assign(N, 0);
+    for (int z = 0; z < H; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) {
+                const int i = ref_site(L, x, y, z);
+                const int64_t ii = internal_index(L, x, y, z);
+                const float kxp = Kx[i], kxm = Kx[ref_site(L, (x + L - 1) % L, y, z)];
+                const float kyp = Ky[i], kym = Ky[ref_site(L, x, (y + L - 1) % L, z)];
+                const double sK = (((double)kxp + (double)kxm) + (double)kyp) + (double)kym;
+                kd[0][ii] = (float)((-sK - 4.0 * cc.J2) - 2.0 * cc.J3);
+                kd[1][ii] = 2.0f * kxp;
+                kd[2][ii] = 2.0f * kxm;
+                kd[3][ii] = 2.0f * kyp;
+                kd[4][ii] = 2.0f * kym;
+                p64[ii] = p[i];
+                q[ii] = (int64_t)std::llrint(p[i] * 0x1p32);
+            }
+}
+
+int upload_site_planes(Engine &E, int conf, const double *p)
+{
+    const ConfDesc &cd = E.confs[conf];
+    std::vector<float> kd[5];
+    std::vector<double> p64;
+    std::vector<int64_t> q;
+    build_site_planes(cd, E.cconst[conf], p, kd, p64, q);
+    for (int P = 0; P < 5; ++P)
+        HIPCHK(hipMemcpyAsync(const_cast<float *>(E.st.kd) + P * E.nsites + cd.site_off, kd[P].data(),
+                              sizeof(float) * cd.N, hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(const_cast<double *>(E.st.p64) + cd.site_off, p64.data(), sizeof(double) * cd.N,
+                          hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(const_cast<int64_t *>(E.st.q) + cd.site_off, q.data(), sizeof(int64_t) * cd.N,
+                          hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
+template <typename T>
+int h2d(Engine &E, T *dst, const std::vector<T> &src)
+{
+    HIPCHK(hipMemcpyAsync(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice, E.stream));
+    return 0;
+}
+
+template <typename T>
+int d2h(Engine &E, std::vector<T> &dst, const T *src, int64_t n)
+{
+    dst.resize((size_t)n);
+    HIPCHK(hipMemcpyAsync(dst.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost, E.stream));
+    return 0;
+}
+
+// Zero the measurement accumulators of one config (start of Pt_measurement_loop,
+// J1J2_6tv.f90:1375-1399).
+int reset_measurement(Engine &E, int conf)
+{
+    const ConfDesc &cd = E.confs[conf];
+    const int nrep = cd.nrep, sb = cd.slot_base;
+    HIPCHK(hipMemsetAsync(E.st.sums + (int64_t)sb * NSUMS, 0, sizeof(double) * nrep * NSUMS, E.stream));
+    int rc;
+    if ((rc = h2d(E, E.st.best + sb, std::vector<double>(nrep, 1.7976931348623157e308)))) return rc;
+    HIPCHK(hipMemsetAsync(E.st.final_en + sb, 0, sizeof(double) * nrep, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.hist + cd.hist_off, 0, sizeof(int64_t) * nrep * E.common.bins, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.hmin + sb, 0, sizeof(double) * nrep, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.hmax + sb, 0, sizeof(double) * nrep, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.hbin + sb, 0, sizeof(double) * nrep, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.vcold + sb, 0, nrep, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.vhot + sb, 0, nrep, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.nmeasure + conf, 0, sizeof(int64_t), E.stream));
+    HIPCHK(hipMemsetAsync(E.st.locspin + cd.site_off, 0, sizeof(int32_t) * cd.N, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.locnem + cd.site_off, 0, sizeof(int32_t) * cd.N, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.locsamples + conf, 0, sizeof(int64_t), E.stream));
+    HIPCHK(hipMemsetAsync(E.st.series_len + conf, 0, sizeof(int64_t), E.stream));
+    return 0;
+}
+
+// Initialize_pt_state (J1J2_6tv.f90:1218-1247): identity maps, cleared PT
+// diagnostics, flip counts and FP32/FP64 mismatch counters.
+int reset_config(Engine &E, int conf)
+{
+    const ConfDesc &cd = E.confs[conf];
+    const int nrep = cd.nrep, sb = cd.slot_base;
+    std::vector<int32_t> id(nrep);
+    for (int r = 0; r < nrep; ++r) id[r] = r;
+    int rc;
+    if ((rc = h2d(E, E.st.rep_at_temp + sb, id))) return rc;
+    if ((rc = h2d(E, E.st.temp_of_rep + sb, id))) return rc;
+    if (nrep > 1) {
+        HIPCHK(hipMemsetAsync(E.st.att + cd.pair_base, 0, sizeof(int64_t) * (nrep - 1), E.stream));
+        HIPCHK(hipMemsetAsync(E.st.acc + cd.pair_base, 0, sizeof(int64_t) * (nrep - 1), E.stream));
+    }
+    if (E.st.flips) HIPCHK(hipMemsetAsync(E.st.flips + cd.flip_off, 0, sizeof(int32_t) * nrep * cd.N, E.stream));
+    HIPCHK(hipMemsetAsync(E.st.shadow + 2 * conf, 0, 2 * sizeof(unsigned long long), E.stream));
+    return reset_measurement(E, conf);
+}
+
+int set_ladder(Engine &E, int conf, const double *t)
+{
+    const ConfDesc &cd = E.confs[conf];
+    std::vector<double> beta(cd.nrep);
+    std::vector<float> b2(cd.nrep);
+    for (int s = 0; s < cd.nrep; ++s) {
+        if (!(t[s] > 0.0)) return fail(J1J2_EINVAL, "temperatures must be positive");
+        beta[s] = 1.0 / t[s];
+        b2[s] = (float)(2.0 * beta[s] * LOG2E);
+    }
+    E.temps[conf].assign(t, t + cd.nrep);
+    int rc;
+    if ((rc = h2d(E, const_cast<double *>(E.st.beta) + cd.slot_base, beta))) return rc;
+    if ((rc = h2d(E, const_cast<float *>(E.st.b2) + cd.slot_base, b2))) return rc;
+    HIPCHK(hipStreamSynchronize(E.stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- schedule
+int timed(Engine &E, int kind, hipError_t (*fn)(Engine &, const void *), const void *arg)
+{
+    if (!E.timing) {
+        HIPCHK(fn(E, arg));
+        return 0;
+    }
+    Timer &T = E.timers[kind];
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    if (!T.pool.empty()) {
+        ev = T.pool.back();
+        T.pool.pop_back();
+    } else {
+        HIPCHK(hipEventCreate(&ev.first));
+        HIPCHK(hipEventCreate(&ev.second));
+    }
+    HIPCHK(hipEventRecord(ev.first, E.stream));
+    HIPCHK(fn(E, arg));
+    HIPCHK(hipEventRecord(ev.second, E.stream));
+    T.live.push_back(ev);
+    return 0;
+}
+
+int collect_timers(Engine &E)
+{
+    for (auto &T : E.timers) {
+        for (auto &ev : T.live) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev.first, ev.second));
+            T.total_ms += ms;
+            T.launches += 1;
+            T.pool.push_back(ev);
+        }
+        T.live.clear();
+    }
+    return 0;
+}
+
+struct SweepArg { int g; int k; int c; };
+
+int enqueue_cycle(Engine &E, int mode)
+{
+    // mode bit 0: measure; bit 1: values only (no sweeps, no swaps)
+    const bool measure = mode & 1, values_only = mode & 2;
+    int rc;
+    if (!values_only) {
+        if (E.all_resident) {
+            auto fn = [](Engine &e, const void *) {
+                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.shadow, e.stream);
+            };
+            if ((rc = timed(E, 0, fn, nullptr))) return rc;
+        } else {
+            for (int g = 0; g < (int)E.groups.size(); ++g) {
+                const Group &G = E.groups[g];
+                if (G.resident) {
+                    auto fn = [](Engine &e, const void *a) {
+                        const Group &gg = e.groups[*static_cast<const int *>(a)];
+                        return launch_sweep_resident(e.st, gg.lat0, gg.nlat, gg.N, e.shadow, e.stream);
+                    };
+                    if ((rc = timed(E, 0, fn, &g))) return rc;
+                    continue;
+                }
+                for (int k = 0; k < E.common.ptswapever; ++k)
+                    for (int c = 0; c < 4; ++c) {
+                        SweepArg a{g, k, c};
+                        auto fn = [](Engine &e, const void *p) {
+                            const SweepArg &s = *static_cast<const SweepArg *>(p);
+                            const Group &gg = e.groups[s.g];
+                            if (e.common.precision == J1J2_FP64_PLAIN)
+                                return launch_sweep_plain64(e.st, gg.lat0, gg.nlat, gg.p.L, gg.p.H, s.k, s.c,
+                                                            e.stream);
+                            return launch_sweep_global(e.st, gg.lat0, gg.nlat, gg.p.L, gg.p.H, s.k, s.c,
+                                                       e.shadow, e.stream);
+                        };
+                        if ((rc = timed(E, 0, fn, &a))) return rc;
+                    }
+            }
+        }
+    }
+    for (int g = 0; g < (int)E.groups.size(); ++g) {
+        if (E.groups[g].resident && !values_only) continue;
+        auto fn = [](Engine &e, const void *a) {
+            const Group &gg = e.groups[*static_cast<const int *>(a)];
+            return launch_observables(e.st, gg.lat0, gg.nlat, e.stream);
+        };
+        if ((rc = timed(E, 2, fn, &g))) return rc;
+    }
+    auto ptfn = [](Engine &e, const void *a) { return launch_pt_step(e.st, *static_cast<const int *>(a), e.stream); };
+    const int ptmode = values_only ? 2 : (measure ? 1 : 0);
+    if ((rc = timed(E, 1, ptfn, &ptmode))) return rc;
+    if (values_only) return 0;
+    if (measure && E.locnem) {
+        auto fn = [](Engine &e, const void *) { return launch_locnem(e.st, e.max_n, e.stream); };
+        if ((rc = timed(E, 3, fn, nullptr))) return rc;
+    }
+    HIPCHK(launch_tick(E.st, E.stream));
+    return 0;
+}
+
+int build_graph(Engine &E, int measure)
+{
+    if (E.graph[measure]) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_cycle(E, measure);
+    hipGraph_t cap = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(E.stream, &cap);
+    if (rc) {
+        if (cap) (void)hipGraphDestroy(cap);
+        return rc;
+    }
+    HIPCHK(e2);
+    g = cap;
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e3);
+    E.graph[measure] = ex;
+    return 0;
+}
+
+int run_cycles(Engine &E, int phase, int ncycles, int begin)
+{
+    if (phase != J1J2_PHASE_EQUIL && phase != J1J2_PHASE_MEAS) return fail(J1J2_EINVAL, "unknown phase");
+    const int measure = phase == J1J2_PHASE_MEAS ? 1 : 0;
+    int rc;
+    if (begin) {
+        HIPCHK(launch_begin_phase(E.st, E.stream));
+        if (measure)
+            for (int c = 0; c < (int)E.confs.size(); ++c)
+                if ((rc = reset_measurement(E, c))) return rc;
+    }
+    const bool graph = E.use_graph && !E.timing;
+    if (graph && ncycles > 0 && (rc = build_graph(E, measure))) return rc;
+    for (int i = 0; i < ncycles; ++i) {
+        if (graph) HIPCHK(hipGraphLaunch(E.graph[measure], E.stream));
+        else if ((rc = enqueue_cycle(E, measure))) return rc;
+        E.cycles_done += 1;
+    }
+    HIPCHK(hipStreamSynchronize(E.stream));
+    if (E.timing && (rc = collect_timers(E))) return rc;
+    if (ncycles > 0) E.observed = true;
+    return 0;
+}
+
+// Observables of the current state without advancing it (NMESS <= 0 branch
+// of Pt_measurement_loop, J1J2_6tv.f90:1414-1424).
+int refresh_values(Engine &E)
+{
+    int rc;
+    if ((rc = enqueue_cycle(E, 2))) return rc;
+    HIPCHK(hipStreamSynchronize(E.stream));
+    E.observed = true;
+    return 0;
+}
+
+Engine *as_engine(void *h) { return static_cast<Engine *>(h); }
+
+int check_conf(Engine *E, int32_t conf)
+{
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (conf < 0 || conf >= (int32_t)E->confs.size()) return fail(J1J2_EINVAL, "config index out of range");
+    return 0;
+}
+
+void to_internal(int L, int H, const int8_t *ref, uint8_t *out)
+{
+    for (int z = 0; z < H; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) out[internal_index(L, x, y, z)] = ref[ref_site(L, x, y, z)] > 0 ? 1 : 0;
+}
+
+template <typename S, typename D>
+void from_internal(int L, int H, const S *in, D *ref, D (*conv)(S))
+{
+    for (int z = 0; z < H; ++z)
+        for (int y = 0; y < L; ++y)
+            for (int x = 0; x < L; ++x) ref[ref_site(L, x, y, z)] = conv(in[internal_index(L, x, y, z)]);
+}
+
+int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **handle)
+{
+    if (!handle) return fail(J1J2_EINVAL, "null handle pointer");
+    *handle = nullptr;
+    if (!groups || ngroups < 1) return fail(J1J2_EINVAL, "need at least one parameter group");
+    int rc;
+    for (int g = 0; g < ngroups; ++g)
+        if ((rc = validate_group(groups[g]))) return rc;
+    const j1j2_params &c0 = groups[0];
+    for (int g = 1; g < ngroups; ++g) {
+        const j1j2_params &p = groups[g];
+        if (p.ptswapever != c0.ptswapever || p.neq != c0.neq || p.nmess != c0.nmess || p.bins != c0.bins ||
+            p.sampler != c0.sampler || p.precision != c0.precision || p.flags != c0.flags ||
+            p.ptmeasindex != c0.ptmeasindex || p.seed != c0.seed)
+            return fail(J1J2_EINVAL, "groups must share PTSWAPEVER, PTMEASINDEX, NEQ, NMESS, BINS, sampler, "
+                                     "precision, flags and seed");
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(J1J2_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+
+    Engine *E = new Engine();
+    std::unique_ptr<Engine> guard(E);
+    E->device = device;
+    E->common = c0;
+    HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    E->shadow = (c0.flags & J1J2_FLAG_SHADOW_FP64) && c0.precision == J1J2_FP32_BATCHED;
+    E->locnem = (c0.flags & J1J2_FLAG_WRITELOCNEM) != 0;
+    E->use_graph = !(c0.flags & J1J2_FLAG_NO_GRAPH);
+    const bool flips = (c0.flags & J1J2_FLAG_WRITENUMFLIPS) != 0;
+    const size_t lds_limit = resident_lds_limit();
+
+    int64_t nlat = 0;
+    int64_t series_cap = c0.sampler == J1J2_SAMPLER_LOCAL ? std::max(c0.nmess, 1) : 0;
+    E->all_resident = true;
+    for (int g = 0; g < ngroups; ++g) {
+        const j1j2_params &p = groups[g];
+        Group G;
+        G.p = p;
+        G.N = p.L * p.L * p.H;
+        G.conf0 = (int32_t)E->confs.size();
+        G.lat0 = (int32_t)nlat;
+        G.nlat = p.nconf * p.nreplica;
+        G.resident = p.precision == J1J2_FP32_BATCHED && (p.L % 8) == 0 && (size_t)G.N <= lds_limit &&
+                     !flips && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
+        E->all_resident = E->all_resident && G.resident;
+        E->max_n = std::max(E->max_n, G.N);
+        std::vector<double> t(p.nreplica);
+        temperature_ladder(p.ladder, p.nreplica, p.tmin, p.tmax, p.dt, p.tladdermid, p.tladderfrac, t.data());
+        for (int i = 0; i < p.nconf; ++i) {
+            ConfDesc cd{};
+            cd.L = p.L; cd.H = p.H; cd.N = G.N; cd.nrep = p.nreplica;
+            cd.lat0 = (int32_t)nlat;
+            cd.slot_base = (int32_t)E->nslots;
+            cd.pair_base = (int32_t)E->npairs;
+            cd.iconf = p.iconf0 + i;
+            cd.site_off = E->nsites;
+            cd.hist_off = E->nhist;
+            cd.series_off = E->nseries;
+            cd.flip_off = E->nflips;
+            ConfConst cc{};
+            cc.J1h = p.J1 + p.deltaJ1; cc.J1v = p.J1 - p.deltaJ1; cc.J2 = p.J2; cc.J3 = p.J3;
+            cc.j2x2 = (float)(2.0 * p.J2); cc.j3x2 = (float)(2.0 * p.J3);
+            for (int r = 0; r < p.nreplica; ++r) {
+                LatDesc d{};
+                d.L = p.L; d.H = p.H; d.N = G.N; d.conf = (int32_t)E->confs.size(); d.rep = r;
+                d.slot_base = cd.slot_base; d.spin_off = E->spin_bytes; d.site_off = cd.site_off;
+                E->lats.push_back(d);
+                E->spin_bytes += ((int64_t)G.N + 255) / 256 * 256;
+                ++nlat;
+            }
+            E->nslots += p.nreplica;
+            E->npairs += std::max(p.nreplica - 1, 0);
+            E->nsites += ((int64_t)G.N + 63) / 64 * 64;
+            E->nhist += (int64_t)p.nreplica * p.bins;
+            E->nseries += (int64_t)p.nreplica * 5 * series_cap;
+            if (flips) E->nflips += (int64_t)p.nreplica * G.N;
+            E->confs.push_back(cd);
+            E->cconst.push_back(cc);
+            E->temps.push_back(t);
+        }
+        E->groups.push_back(G);
+    }
+    E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_resident ? 1 : 2);
+    if (E->all_resident || std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.resident; }))
+        HIPCHK(setup_resident((size_t)E->max_n));
+
+    DeviceState &st = E->st;
+    const int64_t nconf = (int64_t)E->confs.size();
+    LatDesc *lats; ConfDesc *confs; ConfConst *cconst;
+    float *kd, *b2; double *p64, *beta; int64_t *q;
+    if ((rc = E->alloc(lats, nlat)) || (rc = E->alloc(confs, nconf)) || (rc = E->alloc(cconst, nconf)) ||
+        (rc = E->alloc(st.spins, E->spin_bytes)) || (rc = E->alloc(kd, 5 * E->nsites)) ||
+        (rc = E->alloc(p64, E->nsites)) || (rc = E->alloc(q, E->nsites)) ||
+        (rc = E->alloc(st.obs, nlat * OBS_N)) || (rc = E->alloc(st.rep_at_temp, E->nslots)) ||
+        (rc = E->alloc(st.temp_of_rep, E->nslots)) || (rc = E->alloc(beta, E->nslots)) ||
+        (rc = E->alloc(b2, E->nslots)) || (rc = E->alloc(st.clk, 1)) ||
+        (rc = E->alloc(st.sums, E->nslots * NSUMS)) || (rc = E->alloc(st.best, E->nslots)) ||
+        (rc = E->alloc(st.final_en, E->nslots)) || (rc = E->alloc(st.hist, E->nhist)) ||
+        (rc = E->alloc(st.hmin, E->nslots)) || (rc = E->alloc(st.hmax, E->nslots)) ||
+        (rc = E->alloc(st.hbin, E->nslots)) || (rc = E->alloc(st.att, std::max<int64_t>(E->npairs, 1))) ||
+        (rc = E->alloc(st.acc, std::max<int64_t>(E->npairs, 1))) || (rc = E->alloc(st.vcold, E->nslots)) ||
+        (rc = E->alloc(st.vhot, E->nslots)) || (rc = E->alloc(st.nmeasure, nconf)) ||
+        (rc = E->alloc(st.etot, E->nslots)) || (rc = E->alloc(st.vals, E->nslots * 5)) ||
+        (rc = E->alloc(st.locspin, E->nsites)) || (rc = E->alloc(st.locnem, E->nsites)) ||
+        (rc = E->alloc(st.locsamples, nconf)) || (rc = E->alloc(st.series, E->nseries)) ||
+        (rc = E->alloc(st.series_len, nconf)) || (rc = E->alloc(st.shadow, 2 * nconf)))
+        return rc;
+    st.flips = nullptr;
+    if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
+    st.lats = lats; st.confs = confs; st.cconst = cconst;
+    st.nlat = (int32_t)nlat; st.nconf = (int32_t)nconf; st.nsites = E->nsites;
+    st.kd = kd; st.p64 = p64; st.q = q; st.beta = beta; st.b2 = b2;
+    st.series_cap = series_cap;
+    st.ptswapever = c0.ptswapever; st.ptmeasindex = c0.ptmeasindex; st.bins = c0.bins;
+    st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;
+    st.seed = (uint32_t)c0.seed;
+    if ((rc = h2d(*E, lats, E->lats)) || (rc = h2d(*E, confs, E->confs)) || (rc = h2d(*E, cconst, E->cconst)))
+        return rc;
+    std::vector<double> zero_p((size_t)E->max_n, 0.0);
+    for (int c = 0; c < (int)nconf; ++c) {
+        if ((rc = set_ladder(*E, c, E->temps[c].data()))) return rc;
+        if ((rc = upload_site_planes(*E, c, zero_p.data()))) return rc;
+        if ((rc = reset_config(*E, c))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(E->stream));
+    *handle = guard.release();
+    return 0;
+}
+
+}  // namespace
+
+// ======================================================================
+// exported C-ABI
+// ======================================================================
+extern "C" {
+
+int j1j2_last_error(char *buf, int32_t len)
+{
+    if (!buf || len <= 0) return J1J2_EINVAL;
+    std::strncpy(buf, g_err.c_str(), (size_t)len - 1);
+    buf[len - 1] = '\0';
+    return 0;
+}
+
+int j1j2_ladder(const j1j2_params *p, double *temps)
+{
+    if (!p || !temps) return fail(J1J2_EINVAL, "null argument");
+    int rc;
+    if ((rc = validate_group(*p))) return rc;
+    temperature_ladder(p->ladder, p->nreplica, p->tmin, p->tmax, p->dt, p->tladdermid, p->tladderfrac, temps);
+    return 0;
+}
+
+int j1j2_make_disorder(int32_t L, int32_t H, double xi, int32_t seed, int32_t map_to_uniform, double phi,
+                       double *out)
+{
+    if (!out || L < 1 || H < 1) return fail(J1J2_EINVAL, "invalid disorder request");
+    return corr_field(L, L, H, xi, seed, map_to_uniform != 0, phi, out);
+}
+
+int j1j2_initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int8_t *spins)
+{
+    if (!spins || L < 1 || H < 1 || start < 0 || start > 2 || straxis < 0 || straxis > 1)
+        return fail(J1J2_EINVAL, "invalid initial-state request");
+    initial_spins(L, H, start, straxis, seed, spins);
+    return 0;
+}
+
+int j1j2_create(const j1j2_params *p, int32_t device, void **handle) { return create(p, 1, device, handle); }
+
+int j1j2_create_batch(const j1j2_params *groups, int32_t ngroups, int32_t device, void **handle)
+{
+    return create(groups, ngroups, device, handle);
+}
+
+void j1j2_destroy(void *handle) { delete as_engine(handle); }
+
+int j1j2_num_configs(void *handle, int32_t *n)
+{
+    Engine *E = as_engine(handle);
+    if (!E || !n) return fail(J1J2_EINVAL, "null argument");
+    *n = (int32_t)E->confs.size();
+    return 0;
+}
+
+int j1j2_config_shape(void *handle, int32_t conf, int32_t *L, int32_t *H, int32_t *nrep)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    if (L) *L = cd.L;
+    if (H) *H = cd.H;
+    if (nrep) *nrep = cd.nrep;
+    return 0;
+}
+
+int j1j2_upload_disorder(void *handle, int32_t conf, const double *p, int64_t n)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    if (!p || n != cd.N) return fail(J1J2_EINVAL, "disorder array must hold L*L*H values");
+    for (int64_t i = 0; i < n; ++i)
+        if (!(std::fabs(p[i]) < 64.0)) return fail(J1J2_EINVAL, "disorder values must be finite with |p| < 64");
+    HIPCHK(hipSetDevice(E->device));
+    E->observed = false;
+    return upload_site_planes(*E, conf, p);
+}
+
+int j1j2_set_spins(void *handle, int32_t conf, const int8_t *spins, int32_t nrep)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    if (!spins || nrep != cd.nrep) return fail(J1J2_EINVAL, "spins must hold NREPLICA*L*L*H values");
+    for (int64_t i = 0; i < (int64_t)nrep * cd.N; ++i)
+        if (spins[i] != 1 && spins[i] != -1) return fail(J1J2_EINVAL, "spins must be +1 or -1");
+    HIPCHK(hipSetDevice(E->device));
+    std::vector<uint8_t> buf(cd.N);
+    for (int r = 0; r < nrep; ++r) {
+        to_internal(cd.L, cd.H, spins + (int64_t)r * cd.N, buf.data());
+        HIPCHK(hipMemcpyAsync(E->st.spins + E->lats[cd.lat0 + r].spin_off, buf.data(), cd.N,
+                              hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+    }
+    if ((rc = reset_config(*E, conf))) return rc;
+    HIPCHK(hipStreamSynchronize(E->stream));
+    E->observed = false;
+    return 0;
+}
+
+int j1j2_set_temperatures(void *handle, int32_t conf, const double *temps, int32_t nrep)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    if (!temps || nrep != E->confs[conf].nrep) return fail(J1J2_EINVAL, "need NREPLICA temperatures");
+    HIPCHK(hipSetDevice(E->device));
+    return set_ladder(*E, conf, temps);
+}
+
+int j1j2_run_cycles(void *handle, int32_t phase, int32_t ncycles, int32_t begin)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (ncycles < 0) return fail(J1J2_EINVAL, "ncycles must be >= 0");
+    HIPCHK(hipSetDevice(E->device));
+    return run_cycles(*E, phase, ncycles, begin);
+}
+
+int j1j2_run(void *handle, int32_t phase, int32_t ncycles)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (ncycles < 0) ncycles = phase == J1J2_PHASE_MEAS ? E->common.nmess : E->common.neq;
+    HIPCHK(hipSetDevice(E->device));
+    return run_cycles(*E, phase, ncycles, 1);
+}
+
+int j1j2_get_spins(void *handle, int32_t conf, int8_t *spins, int32_t nrep)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    if (!spins || nrep != cd.nrep) return fail(J1J2_EINVAL, "spins must hold NREPLICA*L*L*H values");
+    HIPCHK(hipSetDevice(E->device));
+    std::vector<uint8_t> buf(cd.N);
+    for (int r = 0; r < nrep; ++r) {
+        HIPCHK(hipMemcpyAsync(buf.data(), E->st.spins + E->lats[cd.lat0 + r].spin_off, cd.N,
+                              hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+        from_internal<uint8_t, int8_t>(cd.L, cd.H, buf.data(), spins + (int64_t)r * cd.N,
+                                       [](uint8_t b) -> int8_t { return b ? 1 : -1; });
+    }
+    return 0;
+}
+
+int j1j2_get_maps(void *handle, int32_t conf, int32_t *rep_at_temp, int32_t *temp_of_rep)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    HIPCHK(hipSetDevice(E->device));
+    if (rep_at_temp)
+        HIPCHK(hipMemcpyAsync(rep_at_temp, E->st.rep_at_temp + cd.slot_base, sizeof(int32_t) * cd.nrep,
+                              hipMemcpyDeviceToHost, E->stream));
+    if (temp_of_rep)
+        HIPCHK(hipMemcpyAsync(temp_of_rep, E->st.temp_of_rep + cd.slot_base, sizeof(int32_t) * cd.nrep,
+                              hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+int j1j2_get_observables(void *handle, int32_t conf, double *out)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    if (!out) return fail(J1J2_EINVAL, "null output");
+    HIPCHK(hipSetDevice(E->device));
+    if (!E->observed && (rc = refresh_values(*E))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    std::vector<double> v;
+    if ((rc = d2h(*E, v, E->st.vals + (int64_t)cd.slot_base * 5, (int64_t)cd.nrep * 5))) return rc;
+    HIPCHK(hipStreamSynchronize(E->stream));
+    for (int r = 0; r < cd.nrep; ++r)
+        for (int q = 0; q < 5; ++q) out[(int64_t)q * cd.nrep + r] = v[(size_t)r * 5 + q];
+    return 0;
+}
+
+int j1j2_get_sums(void *handle, int32_t conf, double *sums, int64_t *nmeasure, double *best, double *final_en)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    HIPCHK(hipSetDevice(E->device));
+    if (sums)
+        HIPCHK(hipMemcpyAsync(sums, E->st.sums + (int64_t)cd.slot_base * NSUMS, sizeof(double) * cd.nrep * NSUMS,
+                              hipMemcpyDeviceToHost, E->stream));
+    if (nmeasure)
+        HIPCHK(hipMemcpyAsync(nmeasure, E->st.nmeasure + conf, sizeof(int64_t), hipMemcpyDeviceToHost, E->stream));
+    if (best)
+        HIPCHK(hipMemcpyAsync(best, E->st.best + cd.slot_base, sizeof(double) * cd.nrep, hipMemcpyDeviceToHost,
+                              E->stream));
+    if (final_en)
+        HIPCHK(hipMemcpyAsync(final_en, E->st.final_en + cd.slot_base, sizeof(double) * cd.nrep,
+                              hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+int j1j2_get_series(void *handle, int32_t conf, double *series, int64_t cap, int64_t *len)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    if (!E->st.local) return fail(J1J2_ESTATE, "series are kept by the LOCAL sampler only");
+    const ConfDesc &cd = E->confs[conf];
+    HIPCHK(hipSetDevice(E->device));
+    int64_t n = 0;
+    HIPCHK(hipMemcpy(&n, E->st.series_len + conf, sizeof(int64_t), hipMemcpyDeviceToHost));
+    n = std::min(n, E->st.series_cap);
+    if (len) *len = n;
+    if (!series) return 0;
+    if (cap < n) return fail(J1J2_EINVAL, "series buffer too small");
+    std::vector<double> buf;
+    if ((rc = d2h(*E, buf, E->st.series + cd.series_off, (int64_t)cd.nrep * 5 * E->st.series_cap))) return rc;
+    HIPCHK(hipStreamSynchronize(E->stream));
+    for (int r = 0; r < cd.nrep; ++r)
+        for (int q = 0; q < 5; ++q)
+            std::memcpy(series + ((int64_t)r * 5 + q) * n, buf.data() + ((int64_t)r * 5 + q) * E->st.series_cap,
+                        sizeof(double) * n);
+    return 0;
+}
+
+int j1j2_fetch(void *handle, int32_t conf, double *basefields, double *hist, double *hist_min, double *hist_bin,
+               int32_t *swap_att, int32_t *swap_acc, int32_t *visitboth, double *locspin, double *locnem,
+               int32_t *locnem_samples)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    const ConfDesc &cd = E->confs[conf];
+    const int nrep = cd.nrep, sb = cd.slot_base, bins = E->common.bins;
+    HIPCHK(hipSetDevice(E->device));
+    if (!E->observed && (rc = refresh_values(*E))) return rc;
+    std::vector<double> sums, best, fin, hmin, hbin, vals, series;
+    std::vector<int64_t> hcnt, att, acc, nm, ls;
+    std::vector<uint8_t> vc, vh;
+    std::vector<int32_t> lsp, lnm;
+    if ((rc = d2h(*E, sums, E->st.sums + (int64_t)sb * NSUMS, (int64_t)nrep * NSUMS)) ||
+        (rc = d2h(*E, best, E->st.best + sb, nrep)) || (rc = d2h(*E, fin, E->st.final_en + sb, nrep)) ||
+        (rc = d2h(*E, hmin, E->st.hmin + sb, nrep)) || (rc = d2h(*E, hbin, E->st.hbin + sb, nrep)) ||
+        (rc = d2h(*E, vals, E->st.vals + (int64_t)sb * 5, (int64_t)nrep * 5)) ||
+        (rc = d2h(*E, hcnt, E->st.hist + cd.hist_off, (int64_t)nrep * bins)) ||
+        (rc = d2h(*E, att, E->st.att + cd.pair_base, std::max(nrep - 1, 1))) ||
+        (rc = d2h(*E, acc, E->st.acc + cd.pair_base, std::max(nrep - 1, 1))) ||
+        (rc = d2h(*E, nm, E->st.nmeasure + conf, 1)) || (rc = d2h(*E, ls, E->st.locsamples + conf, 1)) ||
+        (rc = d2h(*E, vc, E->st.vcold + sb, nrep)) || (rc = d2h(*E, vh, E->st.vhot + sb, nrep)) ||
+        (rc = d2h(*E, lsp, E->st.locspin + cd.site_off, cd.N)) ||
+        (rc = d2h(*E, lnm, E->st.locnem + cd.site_off, cd.N)))
+        return rc;
+    std::vector<int32_t> rat(nrep);
+    HIPCHK(hipMemcpyAsync(rat.data(), E->st.rep_at_temp + sb, sizeof(int32_t) * nrep, hipMemcpyDeviceToHost,
+                          E->stream));
+    if (E->st.local && (rc = d2h(*E, series, E->st.series + cd.series_off, (int64_t)nrep * 5 * E->st.series_cap)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(E->stream));
+    const int64_t nmeas = nm[0];
+    std::vector<double> hout((size_t)nrep * bins, 0.0);
+    for (int s = 0; s < nrep; ++s) {
+        double o[49];
+        const double beta = 1.0 / E->temps[conf][s];
+        if (E->st.local && nmeas > 0) {
+            int64_t len = std::min<int64_t>(nmeas, E->st.series_cap);
+            const double *sr = series.data() + (int64_t)s * 5 * E->st.series_cap;
+            std::vector<double> h(bins);
+            double hm = 0, hb = 0;
+            process_local_series(sr, sr + E->st.series_cap, sr + 2 * E->st.series_cap, sr + 3 * E->st.series_cap,
+                                 sr + 4 * E->st.series_cap, len, beta, cd.N, bins, o, h.data(), &hm, &hb);
+            for (int b = 0; b < bins; ++b) hout[(size_t)b * nrep + s] = h[b];
+            hmin[s] = hm;
+            hbin[s] = hb;
+        } else if (nmeas > 0) {
+            derive_basefields(&sums[(size_t)s * NSUMS], nmeas, beta, cd.N, best[s], fin[s], o);
+            for (int b = 0; b < bins; ++b)
+                hout[(size_t)b * nrep + s] = (double)hcnt[(size_t)s * bins + b] / (double)nmeas;
+        } else {
+            // NMESS <= 0: instantaneous values of the replica at this slot (J1J2_6tv.f90:1545-1572)
+            const double *v = &vals[(size_t)rat[s] * 5];
+            const double am = v[0], ae = v[1], ax = v[2], ay = v[3], an = v[4];
+            const double ax2 = ax * ax, ay2 = ay * ay, str2 = ax2 + ay2;
+            const double S[NSUMS] = {am, std::fabs(am), am * am, (am * am) * (am * am), ae, ae * ae,
+                                     std::fabs(ax), std::fabs(ay), ax2, ax2 * ax2, ay2, ay2 * ay2, str2,
+                                     str2 * str2, std::sqrt(str2), std::fabs(ay2 - ax2), an, std::fabs(an),
+                                     an * an, (an * an) * (an * an)};
+            derive_basefields(S, 1, beta, cd.N, ae, ae, o);
+        }
+        if (basefields)
+            for (int f = 0; f < 49; ++f) basefields[(int64_t)f * nrep + s] = o[f];
+    }
+    // confhist(slot, bin) is column-major in the reference: slot fastest
+    if (hist) std::memcpy(hist, hout.data(), sizeof(double) * hout.size());
+    if (hist_min) std::memcpy(hist_min, hmin.data(), sizeof(double) * nrep);
+    if (hist_bin) std::memcpy(hist_bin, hbin.data(), sizeof(double) * nrep);
+    for (int s = 0; s + 1 < nrep; ++s) {
+        if (swap_att) swap_att[s] = (int32_t)att[s];
+        if (swap_acc) swap_acc[s] = (int32_t)acc[s];
+    }
+    if (visitboth) {
+        int vb = 0;
+        for (int r = 0; r < nrep; ++r) vb += (vc[r] && vh[r]) ? 1 : 0;
+        *visitboth = vb;
+    }
+    if (locspin) from_internal<int32_t, double>(cd.L, cd.H, lsp.data(), locspin, [](int32_t v) { return (double)v; });
+    if (locnem) from_internal<int32_t, double>(cd.L, cd.H, lnm.data(), locnem, [](int32_t v) { return (double)v; });
+    if (locnem_samples) *locnem_samples = (int32_t)ls[0];
+    return 0;
+}
+
+int j1j2_get_flips(void *handle, int32_t conf, int32_t *flips)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    if (!E->st.flips) return fail(J1J2_ESTATE, "flip counts need J1J2_FLAG_WRITENUMFLIPS");
+    const ConfDesc &cd = E->confs[conf];
+    HIPCHK(hipSetDevice(E->device));
+    std::vector<int32_t> buf;
+    if ((rc = d2h(*E, buf, E->st.flips + cd.flip_off, (int64_t)cd.nrep * cd.N))) return rc;
+    HIPCHK(hipStreamSynchronize(E->stream));
+    for (int s = 0; s < cd.nrep; ++s)
+        from_internal<int32_t, int32_t>(cd.L, cd.H, buf.data() + (int64_t)s * cd.N, flips + (int64_t)s * cd.N,
+                                        [](int32_t v) { return v; });
+    return 0;
+}
+
+int j1j2_get_mismatch(void *handle, int32_t conf, int64_t *proposals, int64_t *mismatches)
+{
+    Engine *E = as_engine(handle);
+    int rc;
+    if ((rc = check_conf(E, conf))) return rc;
+    HIPCHK(hipSetDevice(E->device));
+    unsigned long long v[2] = {0, 0};
+    HIPCHK(hipMemcpy(v, E->st.shadow + 2 * conf, sizeof(v), hipMemcpyDeviceToHost));
+    if (proposals) *proposals = (int64_t)v[0];
+    if (mismatches) *mismatches = (int64_t)v[1];
+    return 0;
+}
+
+int j1j2_set_timing(void *handle, int32_t enable)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    E->timing = enable != 0;
+    for (auto &T : E->timers) {
+        T.total_ms = 0.0;
+        T.launches = 0;
+    }
+    return 0;
+}
+
+int j1j2_get_timing(void *handle, int32_t kind, double *total_ms, int64_t *launches)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (kind < 0 || kind > 3) return fail(J1J2_EINVAL, "timing kind must be 0..3");
+    if (total_ms) *total_ms = E->timers[kind].total_ms;
+    if (launches) *launches = E->timers[kind].launches;
+    return 0;
+}
+
+int j1j2_sweep_path(void *handle, int32_t *path, double *bytes)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (path) *path = E->path;
+    // algorithmic HBM bytes per site update: spin read + write (1 + 1 B) and the
+    // site's disorder value (FP32 4 B or FP64 8 B)
+    if (bytes) *bytes = E->path == 0 ? 10.0 : 6.0;
+    return 0;
+}
+
+int j1j2_synchronize(void *handle)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    HIPCHK(hipSetDevice(E->device));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// kernels.h -- launch interface between the engine's host code and its HIP
+// kernels (kernels.hip).  All launches are asynchronous on the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.h"
+
+namespace j1j2 {
+
+// Per-config coupling constants needed on the device.
+struct ConfConst {
+    double J1h, J1v, J2, J3;
+    float j2x2, j3x2;          // (float)(2*J2), (float)(2*J3)
+};
+
+struct DeviceState {
+    // geometry
+    const LatDesc *lats;       // nlat
+    const ConfDesc *confs;     // nconf
+    const ConfConst *cconst;   // nconf
+    int32_t nlat, nconf;
+    int64_t nsites;            // sum over configs of N (per-site planes)
+    // lattice state
+    uint8_t *spins;            // sum over lattices of N bytes, internal layout
+    const float *kd;           // 5 planes x nsites: kc, 2Kx+, 2Kx-, 2Ky+, 2Ky-
+    const double *p64;         // nsites
+    const int64_t *q;          // nsites, llrint(p * 2^32)
+    int64_t *obs;              // nlat x OBS_N
+    // ladder
+    int32_t *rep_at_temp;      // nslots (per config blocks of nrep)
+    int32_t *temp_of_rep;      // nslots
+    const double *beta;        // nslots
+    const float *b2;           // nslots, (float)(2*beta*log2 e)
+    DeviceClock *clk;
+    // accumulators
+    double *sums;              // nslots x NSUMS
+    double *best, *final_en;   // nslots
+    int64_t *hist;             // sum nrep*bins
+    double *hmin, *hmax, *hbin;
+    int64_t *att, *acc;        // npairs
+    uint8_t *vcold, *vhot;     // nslots (per replica)
+    int64_t *nmeasure;         // nconf
+    double *etot;              // nslots (per replica)
+    double *vals;              // nslots x 5 (per replica m, e, strx, stry, nbar)
+    int32_t *locspin, *locnem; // nsites
+    int64_t *locsamples;       // nconf
+    double *series;            // LOCAL series
+    int64_t *series_len;       // nconf
+    int64_t series_cap;
+    int32_t *flips;            // per slot per site (optional)
+    unsigned long long *shadow;// nconf x 2
+    // run constants
+    int32_t ptswapever, ptmeasindex, bins, local;
+    uint32_t seed;
+};
+
+// LDS-resident FP32 sweep of `ptswapever` sweeps + observables, one block per
+// lattice in lats[lat_begin, lat_begin + nlat).  Requires L % 8 == 0.
+hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat,
+                                 int32_t max_n, bool shadow, hipStream_t s);
+size_t resident_lds_limit();
+// Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).
+hipError_t setup_resident(size_t max_lds);
+// Global-memory FP32 sweep, one colour pass of sweep k of the current cycle
+// for the lattices [lat_begin, lat_begin + nlat) (all of one geometry).
+hipError_t launch_sweep_global(const DeviceState &st, int32_t lat_begin, int32_t nlat,
+                               int32_t L, int32_t H, int32_t k, int32_t colour, bool shadow,
+                               hipStream_t s);
+// FP64 plain sweep in the reference's field order, one colour pass.
+hipError_t launch_sweep_plain64(const DeviceState &st, int32_t lat_begin, int32_t nlat,
+                                int32_t L, int32_t H, int32_t k, int32_t colour, hipStream_t s);
+// Integer observables of every lattice from global memory.
+hipError_t launch_observables(const DeviceState &st, int32_t lat_begin, int32_t nlat, hipStream_t s);
+// Energies, swaps and per-slot accumulation, one block per config.
+// mode bit 0: accumulate measurements; bit 1: energies and values only (no swaps).
+hipError_t launch_pt_step(const DeviceState &st, int32_t mode, hipStream_t s);
+// locspin/locnem accumulation for the replica at slot PTMEASINDEX.
+hipError_t launch_locnem(const DeviceState &st, int32_t max_n, hipStream_t s);
+// clk->cycle += 1
+hipError_t launch_tick(const DeviceState &st, hipStream_t s);
+// clk->phase_start = clk->cycle
+hipError_t launch_begin_phase(const DeviceState &st, hipStream_t s);
+
+}  // namespace j1j2

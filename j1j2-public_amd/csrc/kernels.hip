@@ -1,0 +1,677 @@
+// kernels.hip -- HIP kernels of the J1-J2 Metropolis / parallel-tempering
+// engine for MI355X (gfx950, wave64).
+//
+// Update rule (the oracle restates it on the CPU):
+//   * a sweep is four colour passes, colour(x,y,z) = (x&1) | ((y+z)&1)<<1;
+//     no stencil neighbour of a site (+-x, +-y, +-z, +-x+-y) shares its
+//     colour, so each pass updates N/4 independent sites (needs L, H even);
+//   * the c-th colour's sites are numbered q = (z*L/2 + y/2)*L/2 + x/2; the
+//     uniform for site q in sweep S of replica r of config iconf is word q&3
+//     of Philox4x32-10(counter = (q/4, c | r<<2, S lo, S hi),
+//     key = (seed, iconf)), u24 = word >> 8;
+//   * FP32 form: h = C_i + sum_j 2K_j b_j + 2 J2 D + 2 J3 Z (b in {0,1});
+//     FP64 form: Compute_local_field (J1J2_6tv.f90:2498-2541) in its order;
+//     Metropolis acceptance of Metro_sweep (J1J2_6tv.f90:2572-2604).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace j1j2 {
+
+namespace {
+
+constexpr int RES_THREADS = 512;       // LDS-resident sweep: 8 waves per lattice
+constexpr int GLB_THREADS = 256;
+
+__device__ __forceinline__ uint32_t byte_bit(uint32_t w, int j) { return (w >> (8 * j)) & 1u; }
+__device__ __forceinline__ float byte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xffu); }
+
+// FP32 decision: uphill iff s*h > 0; accept iff not uphill or u24 < 2^24 2^(-b2 s h)
+__device__ __forceinline__ bool decide_f32(float h, bool up, float b2, uint32_t r)
+{
+    float sh = up ? h : -h;
+    float thr = accept_threshold_f32(-(b2 * sh));
+    return !(sh > 0.0f) || ((float)(r >> 8) < thr);
+}
+
+// FP64 decision of Metro_sweep (J1J2_6tv.f90:2572-2604) with the engine's uniform
+__device__ __forceinline__ bool decide_f64(double field, bool up, double beta, uint32_t r)
+{
+    double u = (double)(r >> 8) * 0x1p-24;
+    if (up) {
+        if (field <= 0.0) return true;
+        return u < exp64(-((2.0 * field) * beta));
+    }
+    if (field >= 0.0) return true;
+    return u < exp64((2.0 * field) * beta);
+}
+
+// Compute_local_field (J1J2_6tv.f90:2510-2540), occu = jh = jv = 1.
+// s[k] = spin of neighbour m_{k+1} (+y,-y,+x,-x,+x+y,-x+y,+x-y,-x-y,+z,-z) as +-1;
+// p = (p_i, p(m1), p(m2), p(m3), p(m4)).
+__device__ __forceinline__ double field_f64(const double s[10], const double p[5], const ConfConst &cc)
+{
+    double field = 0.0, n = 0.0;
+    field = field + s[0] * cc.J1v; n = n - s[0]; field = field - p[1] * s[0];
+    field = field + s[1] * cc.J1v; n = n - s[1]; field = field - p[2] * s[1];
+    field = field + s[2] * cc.J1h; n = n + s[2]; field = field + p[3] * s[2];
+    field = field + s[3] * cc.J1h; n = n + s[3]; field = field + p[4] * s[3];
+    field = field + s[4] * cc.J2;
+    field = field + s[5] * cc.J2;
+    field = field + s[6] * cc.J2;
+    field = field + s[7] * cc.J2;
+    field = field + s[8] * cc.J3;
+    field = field + s[9] * cc.J3;
+    field = field + p[0] * n;
+    return field;
+}
+
+__device__ __forceinline__ double pm1(uint32_t b) { return b ? 1.0 : -1.0; }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum of OBS_N int64 values; result valid in thread 0.
+template <int NT>
+__device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < OBS_N; ++k) v[k] = wave_sum<long long>((long long)v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < OBS_N; ++k) red[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < NT / 64; ++w)
+#pragma unroll
+            for (int k = 0; k < OBS_N; ++k) v[k] += red[w][k];
+}
+
+// ---------------------------------------------------------------------------
+// LDS-resident FP32 sweep.  One block per lattice; the lattice's spins (one
+// byte per site, internal layout) live in LDS for all PTSWAPEVER sweeps of the
+// cycle; the per-site FP32 constants stream from L2/HBM as float4 (one per
+// plane per 4 sites).  A thread owns 4 same-colour sites of a half-row (one
+// LDS dword) at a time, so neighbour words are whole dwords: +-y / +-z come
+// from the same half of the adjacent rows, +-x and the diagonals from the
+// other half shifted by one byte (v_alignbyte).
+// ---------------------------------------------------------------------------
+template <int LC, bool SHADOW>
+__device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
+                                              uint32_t *lds32, int64_t (*red)[OBS_N])
+{
+    const int L = LC ? LC : d.L;
+    const int H = d.H;
+    const int half = L >> 1;            // bytes per half-row
+    const int wph = L >> 3;             // dwords per half-row
+    const int tid = threadIdx.x;
+    const int N = L * L * H;
+    uint8_t *gsp = st.spins + d.spin_off;
+
+    for (int i = tid; i < N / 16; i += RES_THREADS)
+        reinterpret_cast<uint4 *>(lds32)[i] = reinterpret_cast<const uint4 *>(gsp)[i];
+
+    const ConfDesc cd = st.confs[d.conf];
+    const ConfConst cc = st.cconst[d.conf];
+    const int slot = st.temp_of_rep[d.slot_base + d.rep];
+    const float b2 = st.b2[d.slot_base + slot];
+    const double beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
+    const uint32_t key0 = st.seed, key1 = (uint32_t)cd.iconf;
+    const uint64_t S0 = st.clk->cycle * (uint64_t)st.ptswapever;
+    const float4 *kc4 = reinterpret_cast<const float4 *>(st.kd + 0 * st.nsites + d.site_off);
+    const float4 *kxp4 = reinterpret_cast<const float4 *>(st.kd + 1 * st.nsites + d.site_off);
+    const float4 *kxm4 = reinterpret_cast<const float4 *>(st.kd + 2 * st.nsites + d.site_off);
+    const float4 *kyp4 = reinterpret_cast<const float4 *>(st.kd + 3 * st.nsites + d.site_off);
+    const float4 *kym4 = reinterpret_cast<const float4 *>(st.kd + 4 * st.nsites + d.site_off);
+    const int nitems = (L * H / 2) * wph;
+    const float rcp_wph = 1.0f / (float)wph, rcp_half = 1.0f / (float)half;
+    unsigned long long sh_prop = 0, sh_mis = 0;
+    __syncthreads();
+
+    for (int k = 0; k < st.ptswapever; ++k) {
+        const uint64_t sweep = S0 + (uint64_t)k;
+#pragma unroll 1
+        for (int c = 0; c < 4; ++c) {
+            const int hx = c & 1, pr = c >> 1;
+            const uint32_t ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
+            for (int t = tid; t < nitems; t += RES_THREADS) {
+                // t = rr*wph + kk, rr = z*half + yh  (exact float division, t < 2^22)
+                const int rr = LC ? t / wph : (int)(((float)t + 0.5f) * rcp_wph);
+                const int kk = t - rr * wph;
+                const int z = LC ? rr / half : (int)(((float)rr + 0.5f) * rcp_half);
+                const int yh = rr - z * half;
+                const int y = 2 * yh + ((pr ^ z) & 1);
+                const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+                const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
+                const int r0 = (z * L + y) * L, rYp = (z * L + yp) * L, rYm = (z * L + ym) * L;
+                const int rZp = (zp * L + y) * L, rZm = (zm * L + y) * L;
+                const int own = (r0 + hx * half) / 4 + kk;                 // dword index
+                const int oh = (1 - hx) * half / 4;                          // other half (dwords)
+                const int kb = hx ? ((kk == wph - 1) ? 0 : kk + 1) : ((kk == 0) ? wph - 1 : kk - 1);
+                const uint32_t S = lds32[own];
+                const uint32_t YP = lds32[(rYp + hx * half) / 4 + kk];
+                const uint32_t YM = lds32[(rYm + hx * half) / 4 + kk];
+                const uint32_t ZP = lds32[(rZp + hx * half) / 4 + kk];
+                const uint32_t ZM = lds32[(rZm + hx * half) / 4 + kk];
+                const uint32_t A0 = lds32[r0 / 4 + oh + kk], B0 = lds32[r0 / 4 + oh + kb];
+                const uint32_t A1 = lds32[rYp / 4 + oh + kk], B1 = lds32[rYp / 4 + oh + kb];
+                const uint32_t A2 = lds32[rYm / 4 + oh + kk], B2 = lds32[rYm / 4 + oh + kb];
+                uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
+                if (hx == 0) {
+                    Xp0 = A0; Xm0 = __builtin_amdgcn_alignbyte(A0, B0, 3);
+                    Xp1 = A1; Xm1 = __builtin_amdgcn_alignbyte(A1, B1, 3);
+                    Xp2 = A2; Xm2 = __builtin_amdgcn_alignbyte(A2, B2, 3);
+                } else {
+                    Xm0 = A0; Xp0 = __builtin_amdgcn_alignbyte(B0, A0, 1);
+                    Xm1 = A1; Xp1 = __builtin_amdgcn_alignbyte(B1, A1, 1);
+                    Xm2 = A2; Xp2 = __builtin_amdgcn_alignbyte(B2, A2, 1);
+                }
+                const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);   // bytewise, <= 4
+                const uint32_t Z = ZP + ZM;                       // bytewise, <= 2
+                const int w4 = (r0 + hx * half) / 4 + kk;         // float4 index of the 4 sites
+                const float4 kc = kc4[w4], kxp = kxp4[w4], kxm = kxm4[w4], kyp = kyp4[w4], kym = kym4[w4];
+                const U4 rn = philox4x32_10((uint32_t)t, ctr1, (uint32_t)sweep, (uint32_t)(sweep >> 32), key0, key1);
+                const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
+                const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
+                const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
+                const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
+                const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
+                const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+                uint32_t flip = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float h = kca[j];
+                    h = __builtin_fmaf(byte_f(Xp0, j), kxpa[j], h);
+                    h = __builtin_fmaf(byte_f(Xm0, j), kxma[j], h);
+                    h = __builtin_fmaf(byte_f(YP, j), kypa[j], h);
+                    h = __builtin_fmaf(byte_f(YM, j), kyma[j], h);
+                    h = __builtin_fmaf(byte_f(D, j), cc.j2x2, h);
+                    h = __builtin_fmaf(byte_f(Z, j), cc.j3x2, h);
+                    const bool up = byte_bit(S, j) != 0;
+                    const bool acc = decide_f32(h, up, b2, ra[j]);
+                    if (SHADOW) {
+                        const int m = 4 * kk + j;                        // position in the half
+                        const int64_t base = d.site_off;
+                        const int mo_p = hx ? ((m == half - 1) ? 0 : m + 1) : m;   // +x in other half
+                        const int mo_m = hx ? m : ((m == 0) ? half - 1 : m - 1);   // -x in other half
+                        double sv[10] = {pm1(byte_bit(YP, j)), pm1(byte_bit(YM, j)), pm1(byte_bit(Xp0, j)),
+                                         pm1(byte_bit(Xm0, j)), pm1(byte_bit(Xp1, j)), pm1(byte_bit(Xm1, j)),
+                                         pm1(byte_bit(Xp2, j)), pm1(byte_bit(Xm2, j)), pm1(byte_bit(ZP, j)),
+                                         pm1(byte_bit(ZM, j))};
+                        double pv[5] = {st.p64[base + r0 + hx * half + m], st.p64[base + rYp + hx * half + m],
+                                        st.p64[base + rYm + hx * half + m],
+                                        st.p64[base + r0 + (1 - hx) * half + mo_p],
+                                        st.p64[base + r0 + (1 - hx) * half + mo_m]};
+                        const bool a64 = decide_f64(field_f64(sv, pv, cc), up, beta, ra[j]);
+                        sh_prop += 1;
+                        sh_mis += (a64 != acc);
+                    }
+                    flip |= (acc ? 1u : 0u) << (8 * j);
+                }
+                lds32[own] = S ^ flip;
+            }
+            __syncthreads();
+        }
+    }
+
+    // ---- observables of the final state (integer sums; phi part fixed point)
+    int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int dpr = L / 4;                   // dwords per row
+    const int64_t *qb = st.q + d.site_off;
+    for (int w = tid; w < N / 4; w += RES_THREADS) {
+        const int row = w / dpr, within = w - row * dpr;
+        const int hx = within >= wph ? 1 : 0, kk = within - hx * wph;
+        const int y = row % L, z = row / L;
+        const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+        const int zp = (z == H - 1) ? 0 : z + 1;
+        const int r0 = row * L, rYp = (z * L + yp) * L, rYm = (z * L + ym) * L, rZp = (zp * L + y) * L;
+        const int oh = (1 - hx) * half / 4;
+        const int kb = hx ? ((kk == wph - 1) ? 0 : kk + 1) : ((kk == 0) ? wph - 1 : kk - 1);
+        const uint32_t S = lds32[w];
+        const uint32_t YP = lds32[(rYp + hx * half) / 4 + kk], YM = lds32[(rYm + hx * half) / 4 + kk];
+        const uint32_t ZP = lds32[(rZp + hx * half) / 4 + kk];
+        const uint32_t A0 = lds32[r0 / 4 + oh + kk], B0 = lds32[r0 / 4 + oh + kb];
+        const uint32_t A1 = lds32[rYp / 4 + oh + kk], B1 = lds32[rYp / 4 + oh + kb];
+        uint32_t Xp0, Xm0, Xp1, Xm1;
+        if (hx == 0) {
+            Xp0 = A0; Xm0 = __builtin_amdgcn_alignbyte(A0, B0, 3);
+            Xp1 = A1; Xm1 = __builtin_amdgcn_alignbyte(A1, B1, 3);
+        } else {
+            Xm0 = A0; Xp0 = __builtin_amdgcn_alignbyte(B0, A0, 1);
+            Xm1 = A1; Xp1 = __builtin_amdgcn_alignbyte(B1, A1, 1);
+        }
+        const int m4 = 2 * __builtin_popcount(S) - 4;
+        v[OBS_M] += m4;
+        v[OBS_SX] += hx ? -m4 : m4;
+        v[OBS_SY] += (y & 1) ? -m4 : m4;
+        v[OBS_BX] += 4 - 2 * __builtin_popcount(S ^ Xp0);
+        v[OBS_BY] += 4 - 2 * __builtin_popcount(S ^ YP);
+        v[OBS_BZ] += 4 - 2 * __builtin_popcount(S ^ ZP);
+        v[OBS_BD] += 8 - 2 * (__builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1));
+        const uint32_t ax = (S ^ Xp0) + (S ^ Xm0), ay = (S ^ YP) + (S ^ YM);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int l = 2 * ((int)((ay >> (8 * j)) & 0xff) - (int)((ax >> (8 * j)) & 0xff));
+            v[OBS_EQ] += qb[4 * w + j] * (int64_t)l;
+        }
+    }
+    if (SHADOW) {
+        sh_prop = wave_sum<unsigned long long>(sh_prop);
+        sh_mis = wave_sum<unsigned long long>(sh_mis);
+        if ((tid & 63) == 0 && sh_prop) {
+            atomicAdd(&st.shadow[2 * d.conf + 0], sh_prop);
+            atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
+        }
+    }
+    for (int i = tid; i < N / 16; i += RES_THREADS)
+        reinterpret_cast<uint4 *>(gsp)[i] = reinterpret_cast<const uint4 *>(lds32)[i];
+    block_reduce_obs<RES_THREADS>(v, red);
+    if (tid == 0) {
+        int64_t *o = st.obs + (int64_t)lat * OBS_N;
+#pragma unroll
+        for (int k = 0; k < OBS_N; ++k) o[k] = v[k];
+    }
+}
+
+template <bool SHADOW>
+__global__ __launch_bounds__(RES_THREADS) void k_sweep_resident(DeviceState st, int32_t lat_begin,
+                                                                int32_t nlat, int32_t per_xcd)
+{
+    extern __shared__ uint32_t lds32[];
+    __shared__ int64_t red[RES_THREADS / 64][OBS_N];
+    // Blocks are dispatched round-robin over the 8 XCDs; give consecutive
+    // lattices (the replicas of one config, which share the disorder planes)
+    // to the same XCD so those planes are served from that XCD's L2.
+    const int b = blockIdx.x;
+    const int li = (b & 7) * per_xcd + (b >> 3);
+    if (li >= nlat) return;
+    const int lat = lat_begin + li;
+    const LatDesc d = st.lats[lat];
+    switch (d.L) {
+        case 16: resident_body<16, SHADOW>(st, d, lat, lds32, red); break;
+        case 24: resident_body<24, SHADOW>(st, d, lat, lds32, red); break;
+        case 32: resident_body<32, SHADOW>(st, d, lat, lds32, red); break;
+        case 48: resident_body<48, SHADOW>(st, d, lat, lds32, red); break;
+        default: resident_body<0, SHADOW>(st, d, lat, lds32, red); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Global-memory sweeps (any even L): one colour pass per launch, one thread
+// per Philox group of 4 same-colour sites, neighbours read from L2/HBM.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void colour_site(int L, int c, int q, int &x, int &y, int &z)
+{
+    const int half = L >> 1;
+    const int xh = q % half, rr = q / half;
+    z = rr / half;
+    const int yh = rr - z * half;
+    x = 2 * xh + (c & 1);
+    y = 2 * yh + (((c >> 1) ^ z) & 1);
+}
+
+struct Nbr {               // internal indices of a site and its ten neighbours
+    int64_t i, m[10];      // m1..m10 = +y,-y,+x,-x,+x+y,-x+y,+x-y,-x-y,+z,-z
+};
+
+__device__ __forceinline__ Nbr neighbours(int L, int H, int x, int y, int z)
+{
+    const int xp = (x == L - 1) ? 0 : x + 1, xm = (x == 0) ? L - 1 : x - 1;
+    const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+    const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
+    Nbr n;
+    n.i = internal_index(L, x, y, z);
+    n.m[0] = internal_index(L, x, yp, z);  n.m[1] = internal_index(L, x, ym, z);
+    n.m[2] = internal_index(L, xp, y, z);  n.m[3] = internal_index(L, xm, y, z);
+    n.m[4] = internal_index(L, xp, yp, z); n.m[5] = internal_index(L, xm, yp, z);
+    n.m[6] = internal_index(L, xp, ym, z); n.m[7] = internal_index(L, xm, ym, z);
+    n.m[8] = internal_index(L, x, y, zp);  n.m[9] = internal_index(L, x, y, zm);
+    return n;
+}
+
+template <int PREC, bool SHADOW>   // PREC 1 = FP32 rule, 0 = FP64 plain rule
+__global__ __launch_bounds__(GLB_THREADS) void k_sweep_global(DeviceState st, int32_t lat_begin,
+                                                              int32_t L, int32_t H, int32_t k, int32_t c)
+{
+    const int lat = lat_begin + blockIdx.y;
+    const LatDesc d = st.lats[lat];
+    const int ncol = (L * L * H) / 4;
+    const int g = blockIdx.x * GLB_THREADS + threadIdx.x;
+    if (4 * g >= ncol) return;
+    const ConfDesc cd = st.confs[d.conf];
+    const ConfConst cc = st.cconst[d.conf];
+    const int slot = st.temp_of_rep[d.slot_base + d.rep];
+    const float b2 = st.b2[d.slot_base + slot];
+    const double beta = st.beta[d.slot_base + slot];
+    const uint64_t sweep = st.clk->cycle * (uint64_t)st.ptswapever + (uint64_t)k;
+    const U4 rn = philox4x32_10((uint32_t)g, (uint32_t)c | ((uint32_t)d.rep << 2), (uint32_t)sweep,
+                                (uint32_t)(sweep >> 32), st.seed, (uint32_t)cd.iconf);
+    const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+    uint8_t *sp = st.spins + d.spin_off;
+    const int64_t so = d.site_off;
+    const float *kc = st.kd + 0 * st.nsites + so, *kxp = st.kd + 1 * st.nsites + so,
+                *kxm = st.kd + 2 * st.nsites + so, *kyp = st.kd + 3 * st.nsites + so,
+                *kym = st.kd + 4 * st.nsites + so;
+    const double *p = st.p64 + so;
+    const int N = L * L * H;
+    unsigned long long sh_prop = 0, sh_mis = 0;
+    for (int j = 0; j < 4; ++j) {
+        const int q = 4 * g + j;
+        if (q >= ncol) break;
+        int x, y, z;
+        colour_site(L, c, q, x, y, z);
+        const Nbr nb = neighbours(L, H, x, y, z);
+        uint32_t b[10];
+#pragma unroll
+        for (int t = 0; t < 10; ++t) b[t] = sp[nb.m[t]];
+        const uint32_t up = sp[nb.i];
+        bool acc, other = false;
+        if (PREC == 1 || SHADOW) {
+            float h = kc[nb.i];
+            h = __builtin_fmaf((float)b[2], kxp[nb.i], h);
+            h = __builtin_fmaf((float)b[3], kxm[nb.i], h);
+            h = __builtin_fmaf((float)b[0], kyp[nb.i], h);
+            h = __builtin_fmaf((float)b[1], kym[nb.i], h);
+            h = __builtin_fmaf((float)(b[4] + b[5] + b[6] + b[7]), cc.j2x2, h);
+            h = __builtin_fmaf((float)(b[8] + b[9]), cc.j3x2, h);
+            const bool a32 = decide_f32(h, up != 0, b2, ra[j]);
+            if (PREC == 1) acc = a32; else other = a32;
+        }
+        if (PREC == 0 || SHADOW) {
+            double sv[10];
+#pragma unroll
+            for (int t = 0; t < 10; ++t) sv[t] = pm1(b[t]);
+            const double pv[5] = {p[nb.i], p[nb.m[0]], p[nb.m[1]], p[nb.m[2]], p[nb.m[3]]};
+            const bool a64 = decide_f64(field_f64(sv, pv, cc), up != 0, beta, ra[j]);
+            if (PREC == 0) acc = a64; else other = a64;
+        }
+        if (SHADOW) { sh_prop += 1; sh_mis += (other != acc); }
+        if (acc) {
+            sp[nb.i] = (uint8_t)(up ^ 1u);
+            if (st.flips) st.flips[cd.flip_off + (int64_t)slot * N + nb.i] += 1;
+        }
+    }
+    if (SHADOW) {
+        sh_prop = wave_sum<unsigned long long>(sh_prop);
+        sh_mis = wave_sum<unsigned long long>(sh_mis);
+        if ((threadIdx.x & 63) == 0 && sh_prop) {
+            atomicAdd(&st.shadow[2 * d.conf + 0], sh_prop);
+            atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Integer observables from global memory (paths without the fused reduction).
+// ---------------------------------------------------------------------------
+constexpr int OBS_THREADS = 256;
+
+__global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int32_t lat_begin)
+{
+    __shared__ int64_t red[OBS_THREADS / 64][OBS_N];
+    const int lat = lat_begin + blockIdx.x;
+    const LatDesc d = st.lats[lat];
+    const int L = d.L, H = d.H, N = d.N, half = L >> 1;
+    const uint8_t *sp = st.spins + d.spin_off;
+    const int64_t *qb = st.q + d.site_off;
+    int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < N; i += OBS_THREADS) {
+        const int row = i / L, within = i - row * L;
+        const int hx = within >= half ? 1 : 0;
+        const int x = 2 * (within - hx * half) + hx, y = row % L, z = row / L;
+        const Nbr nb = neighbours(L, H, x, y, z);
+        const int si = sp[i] ? 1 : -1;
+        int s[10];
+#pragma unroll
+        for (int t = 0; t < 10; ++t) s[t] = sp[nb.m[t]] ? 1 : -1;
+        v[OBS_M] += si;
+        v[OBS_SX] += (x & 1) ? -si : si;
+        v[OBS_SY] += (y & 1) ? -si : si;
+        v[OBS_BX] += si * s[2];
+        v[OBS_BY] += si * s[0];
+        v[OBS_BD] += si * s[4] + si * s[5];
+        v[OBS_BZ] += si * s[8];
+        v[OBS_EQ] += qb[i] * (int64_t)(si * (s[2] + s[3] - s[0] - s[1]));
+    }
+    block_reduce_obs<OBS_THREADS>(v, red);
+    if (threadIdx.x == 0) {
+        int64_t *o = st.obs + (int64_t)lat * OBS_N;
+        for (int k = 0; k < OBS_N; ++k) o[k] = v[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-config PT step (one wave per config): energies from the integer
+// observables, one pass of adjacent swaps (Pt_attempt_swaps, J1J2_6tv.f90:
+// 1303-1338; pairs are disjoint so they are decided in parallel), then the
+// per-slot accumulation of Pt_measurement_loop (J1J2_6tv.f90:1480-1524).
+// ---------------------------------------------------------------------------
+constexpr int PT_THREADS = 64;
+constexpr int MAX_NREP = 256;
+
+__global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t mode)
+{
+    // mode bit 0: accumulate measurements; bit 1: energies and values only
+    const bool measure = (mode & 1) != 0, swaps = (mode & 2) == 0;
+    __shared__ int32_t rat[MAX_NREP], tor[MAX_NREP];
+    __shared__ double et[MAX_NREP], vv[MAX_NREP][5];
+    const int conf = blockIdx.x, tid = threadIdx.x;
+    const ConfDesc cd = st.confs[conf];
+    const ConfConst cc = st.cconst[conf];
+    const int nrep = cd.nrep, sb = cd.slot_base;
+    const double N = (double)cd.N;
+    const uint64_t cycle = st.clk->cycle;
+    const int phase = ((cycle - st.clk->phase_start) & 1ull) ? 2 : 1;
+    const int64_t slen = st.local ? st.series_len[conf] : 0;
+    for (int r = tid; r < nrep; r += PT_THREADS) {
+        const int64_t *o = st.obs + (int64_t)(cd.lat0 + r) * OBS_N;
+        const double ej = ((cc.J1h * (double)o[OBS_BX]) + (cc.J1v * (double)o[OBS_BY])) +
+                          ((cc.J2 * (double)o[OBS_BD]) + (cc.J3 * (double)o[OBS_BZ]));
+        const double E = -ej - (double)o[OBS_EQ] * 0x1p-32;
+        et[r] = E;
+        vv[r][0] = (double)o[OBS_M] / N;
+        vv[r][1] = E / N;
+        vv[r][2] = (double)o[OBS_SX] / N;
+        vv[r][3] = (double)o[OBS_SY] / N;
+        vv[r][4] = (double)(2 * (o[OBS_BX] - o[OBS_BY])) / N;
+        st.etot[sb + r] = E;
+        for (int q = 0; q < 5; ++q) st.vals[(int64_t)(sb + r) * 5 + q] = vv[r][q];
+        rat[r] = st.rep_at_temp[sb + r];
+        tor[r] = st.temp_of_rep[sb + r];
+    }
+    __syncthreads();
+    if (!st.local && swaps) {
+        const int start = (phase == 1) ? 0 : 1;
+        for (int s = start + 2 * tid; s < nrep - 1; s += 2 * PT_THREADS) {
+            const int rlo = rat[s], rhi = rat[s + 1];
+            st.att[cd.pair_base + s] += 1;
+            const double delta = (st.beta[sb + s] - st.beta[sb + s + 1]) * (et[rlo] - et[rhi]);
+            const U4 rn = philox4x32_10((uint32_t)s, 0x80000000u, (uint32_t)cycle, (uint32_t)(cycle >> 32),
+                                        st.seed, (uint32_t)cd.iconf);
+            const double u = (double)rn.x * 0x1p-32;
+            if (delta >= 0.0 || u < exp64(delta)) {
+                st.acc[cd.pair_base + s] += 1;
+                rat[s] = rhi;
+                rat[s + 1] = rlo;
+                tor[rlo] = s + 1;
+                tor[rhi] = s;
+            }
+        }
+        __syncthreads();
+        for (int r = tid; r < nrep; r += PT_THREADS) {
+            st.rep_at_temp[sb + r] = rat[r];
+            st.temp_of_rep[sb + r] = tor[r];
+        }
+    }
+    if (measure) {
+        for (int sl = tid; sl < nrep; sl += PT_THREADS) {
+            if (!st.local) {
+                if (tor[sl] == 0) st.vcold[sb + sl] = 1;           // sl as a replica index
+                if (tor[sl] == nrep - 1) st.vhot[sb + sl] = 1;
+            }
+            const int r = rat[sl];
+            const double am = vv[r][0], ae = vv[r][1], ax = vv[r][2], ay = vv[r][3], an = vv[r][4];
+            if (st.local) {
+                if (slen < st.series_cap) {
+                    double *sr = st.series + cd.series_off + (int64_t)sl * 5 * st.series_cap;
+                    for (int q = 0; q < 5; ++q) sr[(int64_t)q * st.series_cap + slen] = vv[r][q];
+                }
+                continue;
+            }
+            const double ax2 = ax * ax, ay2 = ay * ay, m2 = am * am, e2 = ae * ae, n2 = an * an;
+            const double str2 = ax2 + ay2;
+            double *S = st.sums + (int64_t)(sb + sl) * NSUMS;
+            S[0] += am;      S[1] += fabs(am);      S[2] += m2;         S[3] += m2 * m2;
+            S[4] += ae;      S[5] += e2;            S[6] += fabs(ax);   S[7] += fabs(ay);
+            S[8] += ax2;     S[9] += ax2 * ax2;     S[10] += ay2;       S[11] += ay2 * ay2;
+            S[12] += str2;   S[13] += str2 * str2;  S[14] += sqrt(str2); S[15] += fabs(ay2 - ax2);
+            S[16] += an;     S[17] += fabs(an);     S[18] += n2;        S[19] += n2 * n2;
+            if (ae < st.best[sb + sl]) st.best[sb + sl] = ae;
+            st.final_en[sb + sl] = ae;
+            if (st.hbin[sb + sl] <= 0.0) {
+                st.hmin[sb + sl] = ae - (fabs(ae) + 1.0);
+                st.hmax[sb + sl] = ae + (fabs(ae) + 1.0);
+                st.hbin[sb + sl] = (st.hmax[sb + sl] - st.hmin[sb + sl]) / (double)st.bins;
+            }
+            const double tq = (ae - st.hmin[sb + sl]) / st.hbin[sb + sl];
+            int idx;
+            if (tq < 0.0) idx = 1;
+            else if (tq >= (double)st.bins) idx = st.bins;
+            else idx = 1 + (int)tq;
+            idx = idx < 1 ? 1 : (idx > st.bins ? st.bins : idx);
+            st.hist[cd.hist_off + (int64_t)sl * st.bins + idx - 1] += 1;
+        }
+        if (tid == 0) {
+            st.nmeasure[conf] += 1;
+            if (st.local) st.series_len[conf] = slen + 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) for the replica
+// at slot PTMEASINDEX, as exact integer sums.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_locnem(DeviceState st)
+{
+    const int conf = blockIdx.y;
+    const ConfDesc cd = st.confs[conf];
+    const int r = st.rep_at_temp[cd.slot_base + st.ptmeasindex - 1];
+    const LatDesc d = st.lats[cd.lat0 + r];
+    const int L = d.L, H = d.H, N = d.N, half = L >> 1;
+    const uint8_t *sp = st.spins + d.spin_off;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        const int row = i / L, within = i - row * L;
+        const int hx = within >= half ? 1 : 0;
+        const int x = 2 * (within - hx * half) + hx, y = row % L, z = row / L;
+        const Nbr nb = neighbours(L, H, x, y, z);
+        const int si = sp[i] ? 1 : -1;
+        const int l = si * ((sp[nb.m[2]] ? 1 : -1) - (sp[nb.m[0]] ? 1 : -1) + (sp[nb.m[3]] ? 1 : -1) -
+                            (sp[nb.m[1]] ? 1 : -1));
+        st.locspin[d.site_off + i] += si;
+        st.locnem[d.site_off + i] += l;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.locsamples[conf] += 1;
+}
+
+__global__ void k_tick(DeviceClock *clk) { clk->cycle += 1; }
+__global__ void k_begin_phase(DeviceClock *clk) { clk->phase_start = clk->cycle; }
+
+constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (RES_THREADS / 64) * OBS_N;
+
+}  // namespace
+
+size_t resident_lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    size_t lim = (size_t)v;
+    return lim > RES_STATIC_LDS ? lim - RES_STATIC_LDS : 0;
+}
+
+hipError_t setup_resident(size_t max_lds)
+{
+    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_resident<true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)k_sweep_resident<false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+}
+
+hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t max_n,
+                                 bool shadow, hipStream_t s)
+{
+    if (nlat <= 0) return hipSuccess;
+    const int per_xcd = (nlat + 7) / 8;
+    const dim3 grid(8 * per_xcd), block(RES_THREADS);
+    const size_t lds = (size_t)max_n;
+    if (shadow)
+        hipLaunchKernelGGL(k_sweep_resident<true>, grid, block, lds, s, st, lat_begin, nlat, per_xcd);
+    else
+        hipLaunchKernelGGL(k_sweep_resident<false>, grid, block, lds, s, st, lat_begin, nlat, per_xcd);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_global(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t L, int32_t H,
+                               int32_t k, int32_t colour, bool shadow, hipStream_t s)
+{
+    if (nlat <= 0) return hipSuccess;
+    const int ngroups = ((L * L * H) / 4 + 3) / 4;
+    const dim3 grid((ngroups + GLB_THREADS - 1) / GLB_THREADS, nlat), block(GLB_THREADS);
+    if (shadow)
+        hipLaunchKernelGGL((k_sweep_global<1, true>), grid, block, 0, s, st, lat_begin, L, H, k, colour);
+    else
+        hipLaunchKernelGGL((k_sweep_global<1, false>), grid, block, 0, s, st, lat_begin, L, H, k, colour);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_plain64(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t L, int32_t H,
+                                int32_t k, int32_t colour, hipStream_t s)
+{
+    if (nlat <= 0) return hipSuccess;
+    const int ngroups = ((L * L * H) / 4 + 3) / 4;
+    const dim3 grid((ngroups + GLB_THREADS - 1) / GLB_THREADS, nlat), block(GLB_THREADS);
+    hipLaunchKernelGGL((k_sweep_global<0, false>), grid, block, 0, s, st, lat_begin, L, H, k, colour);
+    return hipGetLastError();
+}
+
+hipError_t launch_observables(const DeviceState &st, int32_t lat_begin, int32_t nlat, hipStream_t s)
+{
+    if (nlat <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_observables, dim3(nlat), dim3(OBS_THREADS), 0, s, st, lat_begin);
+    return hipGetLastError();
+}
+
+hipError_t launch_pt_step(const DeviceState &st, int32_t mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pt_step, dim3(st.nconf), dim3(PT_THREADS), 0, s, st, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_locnem(const DeviceState &st, int32_t max_n, hipStream_t s)
+{
+    const int chunks = (max_n + 256 * 8 - 1) / (256 * 8);
+    hipLaunchKernelGGL(k_locnem, dim3(chunks, st.nconf), dim3(256), 0, s, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_tick(const DeviceState &st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s, st.clk);
+    return hipGetLastError();
+}
+
+hipError_t launch_begin_phase(const DeviceState &st, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_begin_phase, dim3(1), dim3(1), 0, s, st.clk);
+    return hipGetLastError();
+}
+
+}  // namespace j1j2

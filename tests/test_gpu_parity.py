@@ -1,0 +1,151 @@
+"""GPU parity: the engine (through its C-ABI) against the oracle on the same
+seeded inputs -- spins, replica maps, swap counters, moment sums, histograms,
+derived fields and local-nematic sums bit for bit."""
+import numpy as np
+import pytest
+
+import j1j2_engine as J
+import oracle as O
+from gpu_cases import SEED, build_engine, compare_state, oracle_for, say
+
+pytestmark = pytest.mark.gpu
+
+LOCNEM = J.FLAG_WRITELOCNEM
+
+
+def run_and_compare(groups, starts, neq, nmess, ora_configs=None, shadow=False, flips=False):
+    eng, inputs = build_engine(groups, starts)
+    say(f"engine path {J.PATH_NAMES[eng.sweep_path()[0]]}, {eng.nconf} configs")
+    eng.run(J.PHASE_EQUIL, neq)
+    eng.run(J.PHASE_MEAS, nmess)
+    eng.synchronize()
+    checked = 0
+    for c, (g, iconf, p, s) in enumerate(inputs):
+        if ora_configs is not None and c not in ora_configs:
+            continue
+        ora = oracle_for(g, iconf, p, s, shadow=shadow, count_flips=flips)
+        ora.run(neq, False)
+        ora.run(nmess, True)
+        compare_state(eng, c, ora, J.ladder(g), measured=True)
+        if shadow:
+            assert eng.mismatch(c) == ora.shadow
+        if flips:
+            assert np.array_equal(eng.flips(c), ora.flips)
+        checked += 1
+    return eng, checked
+
+
+@pytest.mark.parametrize("L,H,R,start", [(8, 8, 4, "hot"), (16, 8, 3, "str"), (8, 2, 2, "hot")])
+def test_fp32_resident_matches_oracle(L, H, R, start):
+    say(f"case fp32 resident L={L} H={H} R={R} {start}")
+    g = J.make_params(L, H, nreplica=R, nconf=2, iconf0=3, precision=J.FP32_BATCHED, flags=LOCNEM,
+                      seed=SEED, ptmeasindex=min(2, R))
+    eng, n = run_and_compare([g], start, 3, 6)
+    assert eng.sweep_path()[0] == 1 and n == 2
+
+
+@pytest.mark.parametrize("L,H,R", [(6, 4, 3), (2, 4, 2), (10, 6, 4)])
+def test_fp32_global_matches_oracle(L, H, R):
+    say(f"case fp32 global L={L} H={H} R={R}")
+    g = J.make_params(L, H, nreplica=R, nconf=2, precision=J.FP32_BATCHED, flags=LOCNEM, seed=SEED)
+    eng, n = run_and_compare([g], "hot", 3, 5)
+    assert eng.sweep_path()[0] == 2
+
+
+@pytest.mark.parametrize("L,H,R", [(8, 8, 4), (6, 4, 3)])
+def test_fp64_plain_matches_oracle(L, H, R):
+    say(f"case fp64 plain L={L} H={H} R={R}")
+    g = J.make_params(L, H, nreplica=R, nconf=2, precision=J.FP64_PLAIN, flags=LOCNEM, seed=SEED)
+    eng, n = run_and_compare([g], "hot", 3, 5)
+    assert eng.sweep_path()[0] == 0
+
+
+def test_resident_and_global_paths_agree():
+    say("case resident vs global, L=16")
+    outs = []
+    for flags in (0, J.FLAG_FORCE_GLOBAL):
+        g = J.make_params(16, 16, nreplica=4, nconf=3, flags=flags, seed=SEED)
+        eng, _ = build_engine([g], "hot")
+        eng.run(J.PHASE_EQUIL, 4)
+        outs.append([eng.spins(c) for c in range(3)] + [eng.observables(c) for c in range(3)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_graph_and_eager_schedules_agree():
+    say("case graph vs eager")
+    outs = []
+    for flags in (0, J.FLAG_NO_GRAPH):
+        g = J.make_params(8, 8, nreplica=4, nconf=2, flags=flags | LOCNEM, seed=SEED)
+        eng, _ = build_engine([g], "hot")
+        eng.run(J.PHASE_EQUIL, 3)
+        eng.run(J.PHASE_MEAS, 4)
+        outs.append([eng.spins(c) for c in range(2)] + [eng.fetch(c)["basefields"] for c in range(2)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
+def test_uneven_batch_matches_oracle():
+    say("case uneven batch")
+    groups = [J.make_params(L, L, nreplica=R, nconf=2, iconf0=1 + 2 * k, flags=LOCNEM, seed=SEED)
+              for k, (L, R) in enumerate([(8, 2), (16, 3), (24, 4)])]
+    eng, n = run_and_compare(groups, ["hot", "str"] * 3, 2, 3)
+    assert n == 6 and eng.sweep_path()[0] == 1
+
+
+def test_uneven_batch_with_global_group_matches_oracle():
+    say("case uneven batch with a non-resident group")
+    groups = [J.make_params(L, L, nreplica=2, nconf=1, iconf0=1 + k, flags=LOCNEM, seed=SEED)
+              for k, L in enumerate([8, 6])]
+    eng, n = run_and_compare(groups, "hot", 2, 3)
+    assert eng.sweep_path()[0] == 2
+
+
+@pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
+def test_local_sampler_matches_oracle(precision):
+    say(f"case LOCAL precision {precision}")
+    g = J.make_params(8, 8, nreplica=1, ptswapever=1, nconf=1, sampler=J.SAMPLER_LOCAL,
+                      precision=precision, ladder=J.LADDER_LOCAL, tmin=10.0, tmax=10.0, dt=1.0,
+                      neq=20, nmess=50, flags=LOCNEM, seed=SEED)
+    run_and_compare([g], "hot", 20, 50)
+
+
+def test_shadow_mismatch_counts_match_oracle():
+    say("case shadow FP64 mismatch counting")
+    for flags in (J.FLAG_SHADOW_FP64, J.FLAG_SHADOW_FP64 | J.FLAG_FORCE_GLOBAL):
+        g = J.make_params(8, 8, nreplica=3, nconf=2, flags=flags, seed=SEED)
+        run_and_compare([g], "hot", 2, 2, shadow=True)
+
+
+def test_flip_counts_match_oracle():
+    say("case WRITENUMFLIPS")
+    g = J.make_params(8, 4, nreplica=3, nconf=1, flags=J.FLAG_WRITENUMFLIPS, seed=SEED)
+    run_and_compare([g], "hot", 2, 3, flips=True)
+
+
+def test_zero_cycles_report_initial_state(golden_dir):
+    """NEQ = NMESS = 0: the engine reports the HOT starts' energies exactly as
+    the reference's cfen table does (fixture from the reference build)."""
+    import os
+    from refdata import find, read_table
+    say("case NMESS=0 vs reference cfen")
+    g = J.make_params(8, 8, nreplica=4, nconf=3, iconf0=1, seed=SEED)
+    eng, _ = build_engine([g], "hot")
+    en = np.stack([eng.fetch(c)["basefields"][9] for c in range(3)])
+    tab = read_table(find(os.path.join(golden_dir, "ref_hot_init_l8"), "cfen"))
+    assert np.allclose(tab[:, 1], en.mean(0), rtol=0, atol=1e-9)
+
+
+def test_pt_ladder_continues_across_calls():
+    """run_cycles without begin continues the swap alternation and the sums."""
+    say("case chunked runs")
+    g = J.make_params(8, 8, nreplica=4, nconf=1, seed=SEED)
+    eng, inputs = build_engine([g], "hot")
+    eng.run(J.PHASE_EQUIL, 2)
+    eng.run_cycles(J.PHASE_MEAS, 2, begin=True)
+    eng.run_cycles(J.PHASE_MEAS, 3, begin=False)
+    gg, iconf, p, s = inputs[0]
+    ora = oracle_for(gg, iconf, p, s)
+    ora.run(2, False)
+    ora.run(5, True)
+    compare_state(eng, 0, ora, J.ladder(gg))
