@@ -787,7 +787,8 @@ int j1j2_get_series(void *handle, int32_t conf, double *series, int64_t cap, int
     const ConfDesc &cd = E->confs[conf];
     HIPCHK(hipSetDevice(E->device));
     int64_t n = 0;
-    HIPCHK(hipMemcpy(&n, E->st.series_len + conf, sizeof(int64_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&n, E->st.series_len + conf, sizeof(int64_t), hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
     n = std::min(n, E->st.series_cap);
     if (len) *len = n;
     if (!series) return 0;
@@ -911,7 +912,8 @@ int j1j2_get_mismatch(void *handle, int32_t conf, int64_t *proposals, int64_t *m
     if ((rc = check_conf(E, conf))) return rc;
     HIPCHK(hipSetDevice(E->device));
     unsigned long long v[2] = {0, 0};
-    HIPCHK(hipMemcpy(v, E->st.shadow + 2 * conf, sizeof(v), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(v, E->st.shadow + 2 * conf, sizeof(v), hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
     if (proposals) *proposals = (int64_t)v[0];
     if (mismatches) *mismatches = (int64_t)v[1];
     return 0;
