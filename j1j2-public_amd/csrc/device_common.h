@@ -35,9 +35,11 @@ struct ConfDesc {         // one disorder config
     int64_t flip_off;     // flip-count offset (nrep*N)
 };
 
-struct DeviceClock {      // advanced by a one-thread kernel at the end of each cycle
+struct DeviceClock {      // advanced by the last block of the PT step of each cycle
     uint64_t cycle;       // cycles completed since the engine was created
     uint64_t phase_start; // cycle at which the current phase began
+    uint32_t done;        // PT-step blocks finished in the current cycle
+    uint32_t pad;
 };
 
 // Per-lattice integer observables written after every cycle.

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -62,6 +63,7 @@ struct Engine {
     int32_t max_n = 0;
     int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global
     bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
+    int32_t res_nt = 256;                      // threads per block of the resident sweep
     uint64_t cycles_done = 0;
     bool observed = false;                     // observables valid for the current state
     std::vector<void *> allocs;
@@ -307,7 +309,7 @@ int enqueue_cycle(Engine &E, int mode)
     if (!values_only) {
         if (E.all_resident) {
             auto fn = [](Engine &e, const void *) {
-                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.shadow, e.stream);
+                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.res_nt, e.shadow, e.stream);
             };
             if ((rc = timed(E, 0, fn, nullptr))) return rc;
         } else {
@@ -316,7 +318,7 @@ int enqueue_cycle(Engine &E, int mode)
                 if (G.resident) {
                     auto fn = [](Engine &e, const void *a) {
                         const Group &gg = e.groups[*static_cast<const int *>(a)];
-                        return launch_sweep_resident(e.st, gg.lat0, gg.nlat, gg.N, e.shadow, e.stream);
+                        return launch_sweep_resident(e.st, gg.lat0, gg.nlat, gg.N, e.res_nt, e.shadow, e.stream);
                     };
                     if ((rc = timed(E, 0, fn, &g))) return rc;
                     continue;
@@ -350,11 +352,10 @@ int enqueue_cycle(Engine &E, int mode)
     const int ptmode = values_only ? 2 : (measure ? 1 : 0);
     if ((rc = timed(E, 1, ptfn, &ptmode))) return rc;
     if (values_only) return 0;
-    if (measure && E.locnem) {
-        auto fn = [](Engine &e, const void *) { return launch_locnem(e.st, e.max_n, e.stream); };
+    if (measure && E.locnem && !E.st.fuse_locnem) {
+        auto fn = [](Engine &e, const void *) { return launch_locnem(e.st, e.max_n, 0, e.stream); };
         if ((rc = timed(E, 3, fn, nullptr))) return rc;
     }
-    HIPCHK(launch_tick(E.st, E.stream));
     return 0;
 }
 
@@ -397,6 +398,11 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
         if (graph) HIPCHK(hipGraphLaunch(E.graph[measure], E.stream));
         else if ((rc = enqueue_cycle(E, measure))) return rc;
         E.cycles_done += 1;
+    }
+    if (E.st.fuse_locnem && ncycles > 0) {
+        // accumulate the state the last measuring cycle left behind
+        HIPCHK(launch_locnem(E.st, E.max_n, 1, E.stream));
+        HIPCHK(hipMemsetAsync(E.st.locnem_pending, 0, sizeof(int32_t) * E.confs.size(), E.stream));
     }
     HIPCHK(hipStreamSynchronize(E.stream));
     if (E.timing && (rc = collect_timers(E))) return rc;
@@ -483,7 +489,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         G.conf0 = (int32_t)E->confs.size();
         G.lat0 = (int32_t)nlat;
         G.nlat = p.nconf * p.nreplica;
-        G.resident = p.precision == J1J2_FP32_BATCHED && (p.L % 8) == 0 && (size_t)G.N <= lds_limit &&
+        G.resident = p.precision == J1J2_FP32_BATCHED && resident_supported(p.L) && (size_t)G.N <= lds_limit &&
                      !flips && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
         E->all_resident = E->all_resident && G.resident;
         E->max_n = std::max(E->max_n, G.N);
@@ -524,6 +530,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         E->groups.push_back(G);
     }
     E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_resident ? 1 : 2);
+    if (const char *nt = std::getenv("J1J2_RES_THREADS")) E->res_nt = std::atoi(nt) == 512 ? 512 : 256;
     if (E->all_resident || std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.resident; }))
         HIPCHK(setup_resident((size_t)E->max_n));
 
@@ -546,7 +553,8 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         (rc = E->alloc(st.etot, E->nslots)) || (rc = E->alloc(st.vals, E->nslots * 5)) ||
         (rc = E->alloc(st.locspin, E->nsites)) || (rc = E->alloc(st.locnem, E->nsites)) ||
         (rc = E->alloc(st.locsamples, nconf)) || (rc = E->alloc(st.series, E->nseries)) ||
-        (rc = E->alloc(st.series_len, nconf)) || (rc = E->alloc(st.shadow, 2 * nconf)))
+        (rc = E->alloc(st.series_len, nconf)) || (rc = E->alloc(st.shadow, 2 * nconf)) ||
+        (rc = E->alloc(st.locnem_pending, nconf)))
         return rc;
     st.flips = nullptr;
     if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
@@ -557,6 +565,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     st.ptswapever = c0.ptswapever; st.ptmeasindex = c0.ptmeasindex; st.bins = c0.bins;
     st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;
     st.seed = (uint32_t)c0.seed;
+    st.fuse_locnem = (E->locnem && E->all_resident) ? 1 : 0;
     if ((rc = h2d(*E, lats, E->lats)) || (rc = h2d(*E, confs, E->confs)) || (rc = h2d(*E, cconst, E->cconst)))
         return rc;
     std::vector<double> zero_p((size_t)E->max_n, 0.0);

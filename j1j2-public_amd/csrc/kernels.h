@@ -46,6 +46,8 @@ struct DeviceState {
     double *vals;              // nslots x 5 (per replica m, e, strx, stry, nbar)
     int32_t *locspin, *locnem; // nsites
     int64_t *locsamples;       // nconf
+    int32_t *locnem_pending;   // nconf: the last cycle measured; its state is still to be accumulated
+    int32_t fuse_locnem;       // 1: the resident sweep accumulates pending local-nematic sums
     double *series;            // LOCAL series
     int64_t *series_len;       // nconf
     int64_t series_cap;
@@ -56,10 +58,12 @@ struct DeviceState {
     uint32_t seed;
 };
 
-// LDS-resident FP32 sweep of `ptswapever` sweeps + observables, one block per
-// lattice in lats[lat_begin, lat_begin + nlat).  Requires L % 8 == 0.
+// LDS-resident FP32 sweep of `ptswapever` sweeps + observables, one block of
+// nt (256 or 512) threads per lattice in lats[lat_begin, lat_begin + nlat).
+// Requires resident_supported(L) for every lattice of the range.
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat,
-                                 int32_t max_n, bool shadow, hipStream_t s);
+                                 int32_t max_n, int32_t nt, bool shadow, hipStream_t s);
+bool resident_supported(int32_t L);
 size_t resident_lds_limit();
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).
 hipError_t setup_resident(size_t max_lds);
@@ -76,10 +80,9 @@ hipError_t launch_observables(const DeviceState &st, int32_t lat_begin, int32_t 
 // Energies, swaps and per-slot accumulation, one block per config.
 // mode bit 0: accumulate measurements; bit 1: energies and values only (no swaps).
 hipError_t launch_pt_step(const DeviceState &st, int32_t mode, hipStream_t s);
-// locspin/locnem accumulation for the replica at slot PTMEASINDEX.
-hipError_t launch_locnem(const DeviceState &st, int32_t max_n, hipStream_t s);
-// clk->cycle += 1
-hipError_t launch_tick(const DeviceState &st, hipStream_t s);
+// locspin/locnem accumulation for the replica at slot PTMEASINDEX (of every
+// config, or with only_pending of the configs whose accumulation is pending).
+hipError_t launch_locnem(const DeviceState &st, int32_t max_n, int32_t only_pending, hipStream_t s);
 // clk->phase_start = clk->cycle
 hipError_t launch_begin_phase(const DeviceState &st, hipStream_t s);
 

@@ -20,7 +20,6 @@ namespace j1j2 {
 
 namespace {
 
-constexpr int RES_THREADS = 512;       // LDS-resident sweep: 8 waves per lattice
 constexpr int GLB_THREADS = 256;
 
 __device__ __forceinline__ uint32_t byte_bit(uint32_t w, int j) { return (w >> (8 * j)) & 1u; }
@@ -93,128 +92,234 @@ __device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N])
             for (int k = 0; k < OBS_N; ++k) v[k] += red[w][k];
 }
 
+// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) over the dwords
+// [w0, N/4) step `stride` of a lattice held in the internal layout (L % 8 == 0).
+// l_i = 2 (a(+y) + a(-y) - a(+x) - a(-x)), a = 1 if the bond is anti-aligned.
+__device__ __forceinline__ void locnem_dwords(const uint32_t *sp32, int L, int H, int w0, int stride,
+                                              int32_t *locspin, int32_t *locnem)
+{
+    const int wph = L >> 3, dpr = L >> 2, N = L * L * H;
+    for (int w = w0; w < N / 4; w += stride) {
+        const int row = w / dpr, within = w - row * dpr;
+        const int hx = within >= wph ? 1 : 0, kk = within - hx * wph;
+        const int y = row % L, z = row / L;
+        const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+        const int oh = (1 - hx) * wph;
+        const int kb = hx ? ((kk == wph - 1) ? 0 : kk + 1) : ((kk == 0) ? wph - 1 : kk - 1);
+        const uint32_t S = sp32[w];
+        const uint32_t YP = sp32[(z * L + yp) * dpr + hx * wph + kk], YM = sp32[(z * L + ym) * dpr + hx * wph + kk];
+        const uint32_t A0 = sp32[row * dpr + oh + kk], B0 = sp32[row * dpr + oh + kb];
+        const uint32_t Xp = hx ? __builtin_amdgcn_alignbyte(B0, A0, 1) : A0;
+        const uint32_t Xm = hx ? A0 : __builtin_amdgcn_alignbyte(A0, B0, 3);
+        const uint32_t ax = (S ^ Xp) + (S ^ Xm), ay = (S ^ YP) + (S ^ YM);
+        int4 ls = reinterpret_cast<int4 *>(locspin)[w], ln = reinterpret_cast<int4 *>(locnem)[w];
+        int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&ln);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            lsp[j] += (int)((S >> (8 * j)) & 1u) * 2 - 1;
+            lnp[j] += 2 * ((int)((ay >> (8 * j)) & 0xffu) - (int)((ax >> (8 * j)) & 0xffu));
+        }
+        reinterpret_cast<int4 *>(locspin)[w] = ls;
+        reinterpret_cast<int4 *>(locnem)[w] = ln;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // LDS-resident FP32 sweep.  One block per lattice; the lattice's spins (one
 // byte per site, internal layout) live in LDS for all PTSWAPEVER sweeps of the
-// cycle; the per-site FP32 constants stream from L2/HBM as float4 (one per
-// plane per 4 sites).  A thread owns 4 same-colour sites of a half-row (one
-// LDS dword) at a time, so neighbour words are whole dwords: +-y / +-z come
-// from the same half of the adjacent rows, +-x and the diagonals from the
-// other half shifted by one byte (v_alignbyte).
+// cycle, and the per-site FP32 constants stream from L2/HBM as float4 (one per
+// plane per 4 sites).  In a colour pass a thread owns one half-row: the L/2
+// same-colour sites (x of one parity) of a row (y, z), WPH = L/8 LDS dwords.
+// Its neighbours are whole half-rows too: +-y and +-z are the same half of the
+// adjacent rows; +-x and the four diagonals are the other half of rows y,
+// y+1, y-1 shifted by one byte (v_alignbyte, with wrap inside the half-row).
 // ---------------------------------------------------------------------------
-template <int LC, bool SHADOW>
+template <int WPH>
+__device__ __forceinline__ void lds_load(const uint32_t *lds32, int base, uint32_t (&v)[WPH])
+{
+    if constexpr (WPH == 4) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(lds32 + base);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else if constexpr (WPH == 2) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(lds32 + base);
+        v[0] = q.x; v[1] = q.y;
+    } else {
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) v[w] = lds32[base + w];
+    }
+}
+
+template <int WPH>
+__device__ __forceinline__ void lds_store(uint32_t *lds32, int base, const uint32_t (&v)[WPH])
+{
+    if constexpr (WPH == 4) {
+        *reinterpret_cast<uint4 *>(lds32 + base) = uint4{v[0], v[1], v[2], v[3]};
+    } else if constexpr (WPH == 2) {
+        *reinterpret_cast<uint2 *>(lds32 + base) = uint2{v[0], v[1]};
+    } else {
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) lds32[base + w] = v[w];
+    }
+}
+
+// +x / -x neighbour bytes of a half-row from the other half (A) of a row:
+// even sites (HX = 0, x = 2m): +x = odd m, -x = odd m-1;
+// odd sites  (HX = 1, x = 2m+1): -x = even m, +x = even m+1.
+template <int WPH, int HX>
+__device__ __forceinline__ void x_neighbours(const uint32_t (&A)[WPH], uint32_t (&Xp)[WPH], uint32_t (&Xm)[WPH])
+{
+#pragma unroll
+    for (int w = 0; w < WPH; ++w) {
+        if constexpr (HX == 0) {
+            Xp[w] = A[w];
+            Xm[w] = __builtin_amdgcn_alignbyte(A[w], A[(w + WPH - 1) % WPH], 3);
+        } else {
+            Xm[w] = A[w];
+            Xp[w] = __builtin_amdgcn_alignbyte(A[(w + 1) % WPH], A[w], 1);
+        }
+    }
+}
+
+struct SweepCtx {
+    const float4 *kc4, *kxp4, *kxm4, *kyp4, *kym4;
+    const double *p64;     // config's FP64 field (shadow evaluation)
+    float b2, j2x2, j3x2;
+    double beta;
+    uint32_t ctr1, sweep_lo, sweep_hi, key0, key1;
+};
+
+template <int WPH, int HX, bool SHADOW>
+__device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc,
+                                                int u, int z, int y, int H, unsigned long long &sh_prop,
+                                                unsigned long long &sh_mis)
+{
+    constexpr int L = 8 * WPH, DPR = 2 * WPH, HALF = 4 * WPH;
+    constexpr int OWN = HX * WPH, OTH = (1 - HX) * WPH;
+    const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
+    const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
+    const int R0 = (z * L + y) * DPR, RYp = (z * L + yp) * DPR, RYm = (z * L + ym) * DPR;
+    const int RZp = (zp * L + y) * DPR, RZm = (zm * L + y) * DPR;
+    uint32_t S[WPH], YP[WPH], YM[WPH], Z[WPH], T[WPH], A[WPH];
+    uint32_t Xp0[WPH], Xm0[WPH], Xp1[WPH], Xm1[WPH], Xp2[WPH], Xm2[WPH], D[WPH];
+    lds_load<WPH>(lds32, R0 + OWN, S);
+    lds_load<WPH>(lds32, RYp + OWN, YP);
+    lds_load<WPH>(lds32, RYm + OWN, YM);
+    lds_load<WPH>(lds32, RZp + OWN, Z);
+    lds_load<WPH>(lds32, RZm + OWN, T);
+    lds_load<WPH>(lds32, RYp + OTH, A);
+    x_neighbours<WPH, HX>(A, Xp1, Xm1);
+    lds_load<WPH>(lds32, RYm + OTH, A);
+    x_neighbours<WPH, HX>(A, Xp2, Xm2);
+    lds_load<WPH>(lds32, R0 + OTH, A);
+    x_neighbours<WPH, HX>(A, Xp0, Xm0);
+    uint32_t ZM[WPH];
+#pragma unroll
+    for (int w = 0; w < WPH; ++w) {
+        ZM[w] = T[w];
+        Z[w] = Z[w] + T[w];                               // bytewise, <= 2
+        D[w] = (Xp1[w] + Xm1[w]) + (Xp2[w] + Xm2[w]);     // bytewise, <= 4
+    }
+#pragma unroll
+    for (int w = 0; w < WPH; ++w) {
+        const int w4 = R0 + OWN + w;                      // float4 index of these 4 sites
+        const float4 kc = cx.kc4[w4], kxp = cx.kxp4[w4], kxm = cx.kxm4[w4], kyp = cx.kyp4[w4],
+                     kym = cx.kym4[w4];
+        const U4 rn = philox4x32_10((uint32_t)(u * WPH + w), cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+        const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
+        const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
+        const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
+        const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
+        const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
+        const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+        uint32_t flip = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float h = kca[j];
+            h = __builtin_fmaf(byte_f(Xp0[w], j), kxpa[j], h);
+            h = __builtin_fmaf(byte_f(Xm0[w], j), kxma[j], h);
+            h = __builtin_fmaf(byte_f(YP[w], j), kypa[j], h);
+            h = __builtin_fmaf(byte_f(YM[w], j), kyma[j], h);
+            h = __builtin_fmaf(byte_f(D[w], j), cx.j2x2, h);
+            h = __builtin_fmaf(byte_f(Z[w], j), cx.j3x2, h);
+            const bool up = byte_bit(S[w], j) != 0;
+            const bool acc = decide_f32(h, up, cx.b2, ra[j]);
+            if constexpr (SHADOW) {
+                const int m = 4 * w + j;                          // position in the half-row
+                const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
+                const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
+                const double sv[10] = {pm1(byte_bit(YP[w], j)), pm1(byte_bit(YM[w], j)),
+                                       pm1(byte_bit(Xp0[w], j)), pm1(byte_bit(Xm0[w], j)),
+                                       pm1(byte_bit(Xp1[w], j)), pm1(byte_bit(Xm1[w], j)),
+                                       pm1(byte_bit(Xp2[w], j)), pm1(byte_bit(Xm2[w], j)),
+                                       pm1(byte_bit(Z[w] - ZM[w], j)), pm1(byte_bit(ZM[w], j))};
+                const double pv[5] = {cx.p64[4 * (R0 + OWN) + m], cx.p64[4 * (RYp + OWN) + m],
+                                      cx.p64[4 * (RYm + OWN) + m], cx.p64[4 * (R0 + OTH) + mo_p],
+                                      cx.p64[4 * (R0 + OTH) + mo_m]};
+                const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[j]);
+                sh_prop += 1;
+                sh_mis += (a64 != acc);
+            }
+            flip |= (acc ? 1u : 0u) << (8 * j);
+        }
+        S[w] ^= flip;
+    }
+    lds_store<WPH>(lds32, R0 + OWN, S);
+}
+
+template <int WPH, int NT, bool SHADOW>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
                                               uint32_t *lds32, int64_t (*red)[OBS_N])
 {
-    const int L = LC ? LC : d.L;
+    constexpr int L = 8 * WPH, HALF = 4 * WPH;
     const int H = d.H;
-    const int half = L >> 1;            // bytes per half-row
-    const int wph = L >> 3;             // dwords per half-row
     const int tid = threadIdx.x;
     const int N = L * L * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
-    for (int i = tid; i < N / 16; i += RES_THREADS)
+    for (int i = tid; i < N / 16; i += NT)
         reinterpret_cast<uint4 *>(lds32)[i] = reinterpret_cast<const uint4 *>(gsp)[i];
 
     const ConfDesc cd = st.confs[d.conf];
     const ConfConst cc = st.cconst[d.conf];
     const int slot = st.temp_of_rep[d.slot_base + d.rep];
-    const float b2 = st.b2[d.slot_base + slot];
-    const double beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
-    const uint32_t key0 = st.seed, key1 = (uint32_t)cd.iconf;
+    SweepCtx cx;
+    cx.kc4 = reinterpret_cast<const float4 *>(st.kd + 0 * st.nsites + d.site_off);
+    cx.kxp4 = reinterpret_cast<const float4 *>(st.kd + 1 * st.nsites + d.site_off);
+    cx.kxm4 = reinterpret_cast<const float4 *>(st.kd + 2 * st.nsites + d.site_off);
+    cx.kyp4 = reinterpret_cast<const float4 *>(st.kd + 3 * st.nsites + d.site_off);
+    cx.kym4 = reinterpret_cast<const float4 *>(st.kd + 4 * st.nsites + d.site_off);
+    cx.p64 = st.p64 + d.site_off;
+    cx.b2 = st.b2[d.slot_base + slot];
+    cx.j2x2 = cc.j2x2;
+    cx.j3x2 = cc.j3x2;
+    cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
+    cx.key0 = st.seed;
+    cx.key1 = (uint32_t)cd.iconf;
     const uint64_t S0 = st.clk->cycle * (uint64_t)st.ptswapever;
-    const float4 *kc4 = reinterpret_cast<const float4 *>(st.kd + 0 * st.nsites + d.site_off);
-    const float4 *kxp4 = reinterpret_cast<const float4 *>(st.kd + 1 * st.nsites + d.site_off);
-    const float4 *kxm4 = reinterpret_cast<const float4 *>(st.kd + 2 * st.nsites + d.site_off);
-    const float4 *kyp4 = reinterpret_cast<const float4 *>(st.kd + 3 * st.nsites + d.site_off);
-    const float4 *kym4 = reinterpret_cast<const float4 *>(st.kd + 4 * st.nsites + d.site_off);
-    const int nitems = (L * H / 2) * wph;
-    const float rcp_wph = 1.0f / (float)wph, rcp_half = 1.0f / (float)half;
+    const int nunits = HALF * H;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
+    // the previous cycle measured and left this replica at slot PTMEASINDEX:
+    // accumulate its local nematic sums from the state it ended in
+    if (st.fuse_locnem && slot == st.ptmeasindex - 1 && st.locnem_pending[d.conf]) {
+        locnem_dwords(lds32, L, H, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
+        if (tid == 0) st.locsamples[d.conf] += 1;
+        __syncthreads();   // every read of the unswept state precedes the first colour pass
+    }
 
     for (int k = 0; k < st.ptswapever; ++k) {
         const uint64_t sweep = S0 + (uint64_t)k;
+        cx.sweep_lo = (uint32_t)sweep;
+        cx.sweep_hi = (uint32_t)(sweep >> 32);
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
-            const int hx = c & 1, pr = c >> 1;
-            const uint32_t ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
-            for (int t = tid; t < nitems; t += RES_THREADS) {
-                // t = rr*wph + kk, rr = z*half + yh  (exact float division, t < 2^22)
-                const int rr = LC ? t / wph : (int)(((float)t + 0.5f) * rcp_wph);
-                const int kk = t - rr * wph;
-                const int z = LC ? rr / half : (int)(((float)rr + 0.5f) * rcp_half);
-                const int yh = rr - z * half;
+            const int pr = c >> 1;
+            cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
+            for (int u = tid; u < nunits; u += NT) {
+                const int z = u / HALF, yh = u - z * HALF;   // u = z*L/2 + y/2
                 const int y = 2 * yh + ((pr ^ z) & 1);
-                const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
-                const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
-                const int r0 = (z * L + y) * L, rYp = (z * L + yp) * L, rYm = (z * L + ym) * L;
-                const int rZp = (zp * L + y) * L, rZm = (zm * L + y) * L;
-                const int own = (r0 + hx * half) / 4 + kk;                 // dword index
-                const int oh = (1 - hx) * half / 4;                          // other half (dwords)
-                const int kb = hx ? ((kk == wph - 1) ? 0 : kk + 1) : ((kk == 0) ? wph - 1 : kk - 1);
-                const uint32_t S = lds32[own];
-                const uint32_t YP = lds32[(rYp + hx * half) / 4 + kk];
-                const uint32_t YM = lds32[(rYm + hx * half) / 4 + kk];
-                const uint32_t ZP = lds32[(rZp + hx * half) / 4 + kk];
-                const uint32_t ZM = lds32[(rZm + hx * half) / 4 + kk];
-                const uint32_t A0 = lds32[r0 / 4 + oh + kk], B0 = lds32[r0 / 4 + oh + kb];
-                const uint32_t A1 = lds32[rYp / 4 + oh + kk], B1 = lds32[rYp / 4 + oh + kb];
-                const uint32_t A2 = lds32[rYm / 4 + oh + kk], B2 = lds32[rYm / 4 + oh + kb];
-                uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
-                if (hx == 0) {
-                    Xp0 = A0; Xm0 = __builtin_amdgcn_alignbyte(A0, B0, 3);
-                    Xp1 = A1; Xm1 = __builtin_amdgcn_alignbyte(A1, B1, 3);
-                    Xp2 = A2; Xm2 = __builtin_amdgcn_alignbyte(A2, B2, 3);
-                } else {
-                    Xm0 = A0; Xp0 = __builtin_amdgcn_alignbyte(B0, A0, 1);
-                    Xm1 = A1; Xp1 = __builtin_amdgcn_alignbyte(B1, A1, 1);
-                    Xm2 = A2; Xp2 = __builtin_amdgcn_alignbyte(B2, A2, 1);
-                }
-                const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);   // bytewise, <= 4
-                const uint32_t Z = ZP + ZM;                       // bytewise, <= 2
-                const int w4 = (r0 + hx * half) / 4 + kk;         // float4 index of the 4 sites
-                const float4 kc = kc4[w4], kxp = kxp4[w4], kxm = kxm4[w4], kyp = kyp4[w4], kym = kym4[w4];
-                const U4 rn = philox4x32_10((uint32_t)t, ctr1, (uint32_t)sweep, (uint32_t)(sweep >> 32), key0, key1);
-                const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
-                const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
-                const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
-                const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
-                const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
-                const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
-                uint32_t flip = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    float h = kca[j];
-                    h = __builtin_fmaf(byte_f(Xp0, j), kxpa[j], h);
-                    h = __builtin_fmaf(byte_f(Xm0, j), kxma[j], h);
-                    h = __builtin_fmaf(byte_f(YP, j), kypa[j], h);
-                    h = __builtin_fmaf(byte_f(YM, j), kyma[j], h);
-                    h = __builtin_fmaf(byte_f(D, j), cc.j2x2, h);
-                    h = __builtin_fmaf(byte_f(Z, j), cc.j3x2, h);
-                    const bool up = byte_bit(S, j) != 0;
-                    const bool acc = decide_f32(h, up, b2, ra[j]);
-                    if (SHADOW) {
-                        const int m = 4 * kk + j;                        // position in the half
-                        const int64_t base = d.site_off;
-                        const int mo_p = hx ? ((m == half - 1) ? 0 : m + 1) : m;   // +x in other half
-                        const int mo_m = hx ? m : ((m == 0) ? half - 1 : m - 1);   // -x in other half
-                        double sv[10] = {pm1(byte_bit(YP, j)), pm1(byte_bit(YM, j)), pm1(byte_bit(Xp0, j)),
-                                         pm1(byte_bit(Xm0, j)), pm1(byte_bit(Xp1, j)), pm1(byte_bit(Xm1, j)),
-                                         pm1(byte_bit(Xp2, j)), pm1(byte_bit(Xm2, j)), pm1(byte_bit(ZP, j)),
-                                         pm1(byte_bit(ZM, j))};
-                        double pv[5] = {st.p64[base + r0 + hx * half + m], st.p64[base + rYp + hx * half + m],
-                                        st.p64[base + rYm + hx * half + m],
-                                        st.p64[base + r0 + (1 - hx) * half + mo_p],
-                                        st.p64[base + r0 + (1 - hx) * half + mo_m]};
-                        const bool a64 = decide_f64(field_f64(sv, pv, cc), up, beta, ra[j]);
-                        sh_prop += 1;
-                        sh_mis += (a64 != acc);
-                    }
-                    flip |= (acc ? 1u : 0u) << (8 * j);
-                }
-                lds32[own] = S ^ flip;
+                if (c & 1) update_half_row<WPH, 1, SHADOW>(lds32, cx, cc, u, z, y, H, sh_prop, sh_mis);
+                else update_half_row<WPH, 0, SHADOW>(lds32, cx, cc, u, z, y, H, sh_prop, sh_mis);
             }
             __syncthreads();
         }
@@ -222,9 +327,9 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 
     // ---- observables of the final state (integer sums; phi part fixed point)
     int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int dpr = L / 4;                   // dwords per row
+    constexpr int wph = WPH, dpr = 2 * WPH, half = HALF;
     const int64_t *qb = st.q + d.site_off;
-    for (int w = tid; w < N / 4; w += RES_THREADS) {
+    for (int w = tid; w < N / 4; w += NT) {
         const int row = w / dpr, within = w - row * dpr;
         const int hx = within >= wph ? 1 : 0, kk = within - hx * wph;
         const int y = row % L, z = row / L;
@@ -269,9 +374,9 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
             atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
         }
     }
-    for (int i = tid; i < N / 16; i += RES_THREADS)
+    for (int i = tid; i < N / 16; i += NT)
         reinterpret_cast<uint4 *>(gsp)[i] = reinterpret_cast<const uint4 *>(lds32)[i];
-    block_reduce_obs<RES_THREADS>(v, red);
+    block_reduce_obs<NT>(v, red);
     if (tid == 0) {
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
@@ -279,12 +384,12 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
 }
 
-template <bool SHADOW>
-__global__ __launch_bounds__(RES_THREADS) void k_sweep_resident(DeviceState st, int32_t lat_begin,
-                                                                int32_t nlat, int32_t per_xcd)
+template <int NT, bool SHADOW>
+__global__ __launch_bounds__(NT) void k_sweep_resident(DeviceState st, int32_t lat_begin, int32_t nlat,
+                                                       int32_t per_xcd)
 {
     extern __shared__ uint32_t lds32[];
-    __shared__ int64_t red[RES_THREADS / 64][OBS_N];
+    __shared__ int64_t red[NT / 64][OBS_N];
     // Blocks are dispatched round-robin over the 8 XCDs; give consecutive
     // lattices (the replicas of one config, which share the disorder planes)
     // to the same XCD so those planes are served from that XCD's L2.
@@ -294,11 +399,13 @@ __global__ __launch_bounds__(RES_THREADS) void k_sweep_resident(DeviceState st, 
     const int lat = lat_begin + li;
     const LatDesc d = st.lats[lat];
     switch (d.L) {
-        case 16: resident_body<16, SHADOW>(st, d, lat, lds32, red); break;
-        case 24: resident_body<24, SHADOW>(st, d, lat, lds32, red); break;
-        case 32: resident_body<32, SHADOW>(st, d, lat, lds32, red); break;
-        case 48: resident_body<48, SHADOW>(st, d, lat, lds32, red); break;
-        default: resident_body<0, SHADOW>(st, d, lat, lds32, red); break;
+        case 8: resident_body<1, NT, SHADOW>(st, d, lat, lds32, red); break;
+        case 16: resident_body<2, NT, SHADOW>(st, d, lat, lds32, red); break;
+        case 24: resident_body<3, NT, SHADOW>(st, d, lat, lds32, red); break;
+        case 32: resident_body<4, NT, SHADOW>(st, d, lat, lds32, red); break;
+        case 40: resident_body<5, NT, SHADOW>(st, d, lat, lds32, red); break;
+        case 48: resident_body<6, NT, SHADOW>(st, d, lat, lds32, red); break;
+        default: break;   // the host only sends L in {8,...,48}, L % 8 == 0, here
     }
 }
 
@@ -552,38 +659,59 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
             if (st.local) st.series_len[conf] = slen + 1;
         }
     }
+    if (swaps) {
+        if (tid == 0 && st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
+        // the last block to finish advances the cycle clock (every block has
+        // read it before incrementing `done`)
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            const unsigned prev = atomicAdd(&st.clk->done, 1u);
+            if (prev == gridDim.x - 1) {
+                st.clk->done = 0;
+                st.clk->cycle = cycle + 1;
+                __threadfence();
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) for the replica
-// at slot PTMEASINDEX, as exact integer sums.
+// Local nematic accumulation (J1J2_6tv.f90:1009-1025) for the replica at slot
+// PTMEASINDEX, as exact integer sums, from global memory.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_locnem(DeviceState st)
+__global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pending)
 {
     const int conf = blockIdx.y;
+    if (only_pending && !st.locnem_pending[conf]) return;
     const ConfDesc cd = st.confs[conf];
     const int r = st.rep_at_temp[cd.slot_base + st.ptmeasindex - 1];
     const LatDesc d = st.lats[cd.lat0 + r];
     const int L = d.L, H = d.H, N = d.N, half = L >> 1;
     const uint8_t *sp = st.spins + d.spin_off;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
-        const int row = i / L, within = i - row * L;
-        const int hx = within >= half ? 1 : 0;
-        const int x = 2 * (within - hx * half) + hx, y = row % L, z = row / L;
-        const Nbr nb = neighbours(L, H, x, y, z);
-        const int si = sp[i] ? 1 : -1;
-        const int l = si * ((sp[nb.m[2]] ? 1 : -1) - (sp[nb.m[0]] ? 1 : -1) + (sp[nb.m[3]] ? 1 : -1) -
-                            (sp[nb.m[1]] ? 1 : -1));
-        st.locspin[d.site_off + i] += si;
-        st.locnem[d.site_off + i] += l;
+    const int t0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+    if ((L & 7) == 0) {
+        locnem_dwords(reinterpret_cast<const uint32_t *>(sp), L, H, t0, stride, st.locspin + d.site_off,
+                      st.locnem + d.site_off);
+    } else {
+        for (int i = t0; i < N; i += stride) {
+            const int row = i / L, within = i - row * L;
+            const int hx = within >= half ? 1 : 0;
+            const int x = 2 * (within - hx * half) + hx, y = row % L, z = row / L;
+            const Nbr nb = neighbours(L, H, x, y, z);
+            const int si = sp[i] ? 1 : -1;
+            const int l = si * ((sp[nb.m[2]] ? 1 : -1) - (sp[nb.m[0]] ? 1 : -1) + (sp[nb.m[3]] ? 1 : -1) -
+                                (sp[nb.m[1]] ? 1 : -1));
+            st.locspin[d.site_off + i] += si;
+            st.locnem[d.site_off + i] += l;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) st.locsamples[conf] += 1;
 }
 
-__global__ void k_tick(DeviceClock *clk) { clk->cycle += 1; }
 __global__ void k_begin_phase(DeviceClock *clk) { clk->phase_start = clk->cycle; }
 
-constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (RES_THREADS / 64) * OBS_N;
+constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (512 / 64) * OBS_N;
 
 }  // namespace
 
@@ -596,26 +724,38 @@ size_t resident_lds_limit()
     return lim > RES_STATIC_LDS ? lim - RES_STATIC_LDS : 0;
 }
 
-hipError_t setup_resident(size_t max_lds)
+template <int NT>
+hipError_t setup_resident_nt(size_t max_lds)
 {
-    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_resident<true>,
+    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_resident<NT, true>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_sweep_resident<false>,
+    return hipFuncSetAttribute((const void *)k_sweep_resident<NT, false>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
 }
 
+hipError_t setup_resident(size_t max_lds)
+{
+    hipError_t e = setup_resident_nt<256>(max_lds);
+    return e != hipSuccess ? e : setup_resident_nt<512>(max_lds);
+}
+
+bool resident_supported(int32_t L) { return L >= 8 && L <= 48 && (L % 8) == 0; }
+
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t max_n,
-                                 bool shadow, hipStream_t s)
+                                 int32_t nt, bool shadow, hipStream_t s)
 {
     if (nlat <= 0) return hipSuccess;
     const int per_xcd = (nlat + 7) / 8;
-    const dim3 grid(8 * per_xcd), block(RES_THREADS);
+    const dim3 grid(8 * per_xcd);
     const size_t lds = (size_t)max_n;
-    if (shadow)
-        hipLaunchKernelGGL(k_sweep_resident<true>, grid, block, lds, s, st, lat_begin, nlat, per_xcd);
-    else
-        hipLaunchKernelGGL(k_sweep_resident<false>, grid, block, lds, s, st, lat_begin, nlat, per_xcd);
+    if (nt == 512) {
+        if (shadow) hipLaunchKernelGGL((k_sweep_resident<512, true>), grid, dim3(512), lds, s, st, lat_begin, nlat, per_xcd);
+        else hipLaunchKernelGGL((k_sweep_resident<512, false>), grid, dim3(512), lds, s, st, lat_begin, nlat, per_xcd);
+    } else {
+        if (shadow) hipLaunchKernelGGL((k_sweep_resident<256, true>), grid, dim3(256), lds, s, st, lat_begin, nlat, per_xcd);
+        else hipLaunchKernelGGL((k_sweep_resident<256, false>), grid, dim3(256), lds, s, st, lat_begin, nlat, per_xcd);
+    }
     return hipGetLastError();
 }
 
@@ -655,16 +795,10 @@ hipError_t launch_pt_step(const DeviceState &st, int32_t mode, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_locnem(const DeviceState &st, int32_t max_n, hipStream_t s)
+hipError_t launch_locnem(const DeviceState &st, int32_t max_n, int32_t only_pending, hipStream_t s)
 {
-    const int chunks = (max_n + 256 * 8 - 1) / (256 * 8);
-    hipLaunchKernelGGL(k_locnem, dim3(chunks, st.nconf), dim3(256), 0, s, st);
-    return hipGetLastError();
-}
-
-hipError_t launch_tick(const DeviceState &st, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, s, st.clk);
+    const int chunks = (max_n / 4 + 256 * 4 - 1) / (256 * 4);
+    hipLaunchKernelGGL(k_locnem, dim3(chunks, st.nconf), dim3(256), 0, s, st, only_pending);
     return hipGetLastError();
 }
 
