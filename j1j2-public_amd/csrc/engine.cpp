@@ -64,6 +64,7 @@ struct Engine {
     int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global
     bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
     int32_t res_nt = 256;                      // threads per block of the resident sweep
+    int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
     uint64_t cycles_done = 0;
     bool observed = false;                     // observables valid for the current state
     std::vector<void *> allocs;
@@ -309,7 +310,8 @@ int enqueue_cycle(Engine &E, int mode)
     if (!values_only) {
         if (E.all_resident) {
             auto fn = [](Engine &e, const void *) {
-                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.res_nt, e.shadow, e.stream);
+                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.uniform_L, e.res_nt, e.shadow,
+                                             e.stream);
             };
             if ((rc = timed(E, 0, fn, nullptr))) return rc;
         } else {
@@ -318,7 +320,8 @@ int enqueue_cycle(Engine &E, int mode)
                 if (G.resident) {
                     auto fn = [](Engine &e, const void *a) {
                         const Group &gg = e.groups[*static_cast<const int *>(a)];
-                        return launch_sweep_resident(e.st, gg.lat0, gg.nlat, gg.N, e.res_nt, e.shadow, e.stream);
+                        return launch_sweep_resident(e.st, gg.lat0, gg.nlat, gg.N, gg.p.L, e.res_nt, e.shadow,
+                                                     e.stream);
                     };
                     if ((rc = timed(E, 0, fn, &g))) return rc;
                     continue;
@@ -530,6 +533,9 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         E->groups.push_back(G);
     }
     E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_resident ? 1 : 2);
+    E->uniform_L = c0.L;
+    for (const Group &G : E->groups)
+        if (G.p.L != c0.L) E->uniform_L = 0;
     if (const char *nt = std::getenv("J1J2_RES_THREADS")) E->res_nt = std::atoi(nt) == 512 ? 512 : 256;
     if (E->all_resident || std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.resident; }))
         HIPCHK(setup_resident((size_t)E->max_n));

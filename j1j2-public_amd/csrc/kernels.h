@@ -61,8 +61,9 @@ struct DeviceState {
 // LDS-resident FP32 sweep of `ptswapever` sweeps + observables, one block of
 // nt (256 or 512) threads per lattice in lats[lat_begin, lat_begin + nlat).
 // Requires resident_supported(L) for every lattice of the range.
+// L: the common lattice size of the range, or 0 for a range of mixed sizes.
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat,
-                                 int32_t max_n, int32_t nt, bool shadow, hipStream_t s);
+                                 int32_t max_n, int32_t L, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);
 size_t resident_lds_limit();
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).

@@ -199,26 +199,32 @@ __device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx 
     const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
     const int R0 = (z * L + y) * DPR, RYp = (z * L + yp) * DPR, RYm = (z * L + ym) * DPR;
     const int RZp = (zp * L + y) * DPR, RZm = (zm * L + y) * DPR;
-    uint32_t S[WPH], YP[WPH], YM[WPH], Z[WPH], T[WPH], A[WPH];
-    uint32_t Xp0[WPH], Xm0[WPH], Xp1[WPH], Xm1[WPH], Xp2[WPH], Xm2[WPH], D[WPH];
-    lds_load<WPH>(lds32, R0 + OWN, S);
+    // Only the reduced neighbour words stay live: Z = b(+z) + b(-z) and
+    // D = sum of the four diagonal b's, bytewise (<= 2 and <= 4: no carries).
+    uint32_t S[WPH], YP[WPH], YM[WPH], Z[WPH], D[WPH], Xp0[WPH], Xm0[WPH];
+    {
+        uint32_t T[WPH];
+        lds_load<WPH>(lds32, RZp + OWN, Z);
+        lds_load<WPH>(lds32, RZm + OWN, T);
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) Z[w] += T[w];
+    }
+    {
+        uint32_t A[WPH], P[WPH], M[WPH];
+        lds_load<WPH>(lds32, RYp + OTH, A);
+        x_neighbours<WPH, HX>(A, P, M);
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) D[w] = P[w] + M[w];
+        lds_load<WPH>(lds32, RYm + OTH, A);
+        x_neighbours<WPH, HX>(A, P, M);
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) D[w] += P[w] + M[w];
+        lds_load<WPH>(lds32, R0 + OTH, A);
+        x_neighbours<WPH, HX>(A, Xp0, Xm0);
+    }
     lds_load<WPH>(lds32, RYp + OWN, YP);
     lds_load<WPH>(lds32, RYm + OWN, YM);
-    lds_load<WPH>(lds32, RZp + OWN, Z);
-    lds_load<WPH>(lds32, RZm + OWN, T);
-    lds_load<WPH>(lds32, RYp + OTH, A);
-    x_neighbours<WPH, HX>(A, Xp1, Xm1);
-    lds_load<WPH>(lds32, RYm + OTH, A);
-    x_neighbours<WPH, HX>(A, Xp2, Xm2);
-    lds_load<WPH>(lds32, R0 + OTH, A);
-    x_neighbours<WPH, HX>(A, Xp0, Xm0);
-    uint32_t ZM[WPH];
-#pragma unroll
-    for (int w = 0; w < WPH; ++w) {
-        ZM[w] = T[w];
-        Z[w] = Z[w] + T[w];                               // bytewise, <= 2
-        D[w] = (Xp1[w] + Xm1[w]) + (Xp2[w] + Xm2[w]);     // bytewise, <= 4
-    }
+    lds_load<WPH>(lds32, R0 + OWN, S);
 #pragma unroll
     for (int w = 0; w < WPH; ++w) {
         const int w4 = R0 + OWN + w;                      // float4 index of these 4 sites
@@ -244,14 +250,23 @@ __device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx 
             const bool up = byte_bit(S[w], j) != 0;
             const bool acc = decide_f32(h, up, cx.b2, ra[j]);
             if constexpr (SHADOW) {
+                // FP64 rule on the same state and draw; the individual
+                // neighbour words are re-read (no colour-c site changes them)
+                uint32_t A1[WPH], A2[WPH], P1[WPH], M1[WPH], P2[WPH], M2[WPH], ZP[WPH], ZM[WPH];
+                lds_load<WPH>(lds32, RYp + OTH, A1);
+                lds_load<WPH>(lds32, RYm + OTH, A2);
+                x_neighbours<WPH, HX>(A1, P1, M1);
+                x_neighbours<WPH, HX>(A2, P2, M2);
+                lds_load<WPH>(lds32, RZp + OWN, ZP);
+                lds_load<WPH>(lds32, RZm + OWN, ZM);
                 const int m = 4 * w + j;                          // position in the half-row
                 const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
                 const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
                 const double sv[10] = {pm1(byte_bit(YP[w], j)), pm1(byte_bit(YM[w], j)),
                                        pm1(byte_bit(Xp0[w], j)), pm1(byte_bit(Xm0[w], j)),
-                                       pm1(byte_bit(Xp1[w], j)), pm1(byte_bit(Xm1[w], j)),
-                                       pm1(byte_bit(Xp2[w], j)), pm1(byte_bit(Xm2[w], j)),
-                                       pm1(byte_bit(Z[w] - ZM[w], j)), pm1(byte_bit(ZM[w], j))};
+                                       pm1(byte_bit(P1[w], j)), pm1(byte_bit(M1[w], j)),
+                                       pm1(byte_bit(P2[w], j)), pm1(byte_bit(M2[w], j)),
+                                       pm1(byte_bit(ZP[w], j)), pm1(byte_bit(ZM[w], j))};
                 const double pv[5] = {cx.p64[4 * (R0 + OWN) + m], cx.p64[4 * (RYp + OWN) + m],
                                       cx.p64[4 * (RYm + OWN) + m], cx.p64[4 * (R0 + OTH) + mo_p],
                                       cx.p64[4 * (R0 + OTH) + mo_m]};
@@ -384,17 +399,34 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
 }
 
-template <int NT, bool SHADOW>
-__global__ __launch_bounds__(NT) void k_sweep_resident(DeviceState st, int32_t lat_begin, int32_t nlat,
-                                                       int32_t per_xcd)
+// Blocks are dispatched round-robin over the 8 XCDs; consecutive lattices (the
+// replicas of one config, which share the disorder planes) go to the same XCD
+// so those planes are served from that XCD's L2.
+__device__ __forceinline__ int xcd_lattice(int per_xcd) { return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3); }
+
+// One geometry.  At most 128 VGPRs (four waves per SIMD) so that four
+// 256-thread blocks, i.e. four L = 32 lattices (32 KiB of LDS each), stay
+// resident on every CU.
+template <int WPH, int NT, bool SHADOW>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : NT / 64, 8)))
+void k_sweep_resident(DeviceState st, int32_t lat_begin, int32_t nlat, int32_t per_xcd)
 {
     extern __shared__ uint32_t lds32[];
     __shared__ int64_t red[NT / 64][OBS_N];
-    // Blocks are dispatched round-robin over the 8 XCDs; give consecutive
-    // lattices (the replicas of one config, which share the disorder planes)
-    // to the same XCD so those planes are served from that XCD's L2.
-    const int b = blockIdx.x;
-    const int li = (b & 7) * per_xcd + (b >> 3);
+    const int li = xcd_lattice(per_xcd);
+    if (li >= nlat) return;
+    const int lat = lat_begin + li;
+    resident_body<WPH, NT, SHADOW>(st, st.lats[lat], lat, lds32, red);
+}
+
+// Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
+template <int NT, bool SHADOW>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : NT / 64, 8)))
+void k_sweep_resident_mixed(DeviceState st, int32_t lat_begin, int32_t nlat, int32_t per_xcd)
+{
+    extern __shared__ uint32_t lds32[];
+    __shared__ int64_t red[NT / 64][OBS_N];
+    const int li = xcd_lattice(per_xcd);
     if (li >= nlat) return;
     const int lat = lat_begin + li;
     const LatDesc d = st.lats[lat];
@@ -724,38 +756,49 @@ size_t resident_lds_limit()
     return lim > RES_STATIC_LDS ? lim - RES_STATIC_LDS : 0;
 }
 
-template <int NT>
-hipError_t setup_resident_nt(size_t max_lds)
+using ResKernel = void (*)(DeviceState, int32_t, int32_t, int32_t);
+
+template <int NT, bool SHADOW>
+ResKernel resident_kernel(int32_t L)
 {
-    hipError_t e = hipFuncSetAttribute((const void *)k_sweep_resident<NT, true>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)k_sweep_resident<NT, false>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+    switch (L) {
+        case 8: return k_sweep_resident<1, NT, SHADOW>;
+        case 16: return k_sweep_resident<2, NT, SHADOW>;
+        case 24: return k_sweep_resident<3, NT, SHADOW>;
+        case 32: return k_sweep_resident<4, NT, SHADOW>;
+        case 40: return k_sweep_resident<5, NT, SHADOW>;
+        case 48: return k_sweep_resident<6, NT, SHADOW>;
+        default: return k_sweep_resident_mixed<NT, SHADOW>;
+    }
+}
+
+ResKernel pick_resident(int32_t L, int32_t nt, bool shadow)
+{
+    if (nt == 512) return shadow ? resident_kernel<512, true>(L) : resident_kernel<512, false>(L);
+    return shadow ? resident_kernel<256, true>(L) : resident_kernel<256, false>(L);
 }
 
 hipError_t setup_resident(size_t max_lds)
 {
-    hipError_t e = setup_resident_nt<256>(max_lds);
-    return e != hipSuccess ? e : setup_resident_nt<512>(max_lds);
+    for (int L : {8, 16, 24, 32, 40, 48, 0})
+        for (int nt : {256, 512})
+            for (bool sh : {false, true}) {
+                hipError_t e = hipFuncSetAttribute((const void *)pick_resident(L, nt, sh),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+                if (e != hipSuccess) return e;
+            }
+    return hipSuccess;
 }
 
 bool resident_supported(int32_t L) { return L >= 8 && L <= 48 && (L % 8) == 0; }
 
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t max_n,
-                                 int32_t nt, bool shadow, hipStream_t s)
+                                 int32_t L, int32_t nt, bool shadow, hipStream_t s)
 {
     if (nlat <= 0) return hipSuccess;
     const int per_xcd = (nlat + 7) / 8;
-    const dim3 grid(8 * per_xcd);
-    const size_t lds = (size_t)max_n;
-    if (nt == 512) {
-        if (shadow) hipLaunchKernelGGL((k_sweep_resident<512, true>), grid, dim3(512), lds, s, st, lat_begin, nlat, per_xcd);
-        else hipLaunchKernelGGL((k_sweep_resident<512, false>), grid, dim3(512), lds, s, st, lat_begin, nlat, per_xcd);
-    } else {
-        if (shadow) hipLaunchKernelGGL((k_sweep_resident<256, true>), grid, dim3(256), lds, s, st, lat_begin, nlat, per_xcd);
-        else hipLaunchKernelGGL((k_sweep_resident<256, false>), grid, dim3(256), lds, s, st, lat_begin, nlat, per_xcd);
-    }
+    hipLaunchKernelGGL(pick_resident(L, nt, shadow), dim3(8 * per_xcd), dim3(nt == 512 ? 512 : 256),
+                       (size_t)max_n, s, st, lat_begin, nlat, per_xcd);
     return hipGetLastError();
 }
 
