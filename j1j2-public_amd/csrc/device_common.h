@@ -49,11 +49,29 @@ enum { OBS_M = 0, OBS_SX, OBS_SY, OBS_BX, OBS_BY, OBS_BD, OBS_BZ, OBS_EQ, OBS_N 
 // (J1J2_6tv.f90:1493-1512).
 enum { NSUMS = 20 };
 
-// Internal spin layout: row r = z*L + y holds L bytes, the even-x half first
-// ([x=0,2,..,L-2]) then the odd-x half; a byte is b = (s+1)/2.
-__host__ __device__ inline int64_t internal_index(int32_t L, int32_t x, int32_t y, int32_t z)
+// Internal site layout ("colour planes"): the N/4 sites of colour
+// c = (x&1) | ((y+z)&1)<<1 form plane c, ordered by the colour index
+// q = (z*L/2 + y/2)*L/2 + x/2, so a colour pass touches one contiguous plane.
+// Within a plane, unit u = z*L/2 + y/2 is one half-row (L/2 sites with the
+// same x parity).  Spins are stored as one byte b = (s+1)/2 per site.
+__host__ __device__ inline int64_t internal_index(int32_t L, int32_t H, int32_t x, int32_t y, int32_t z)
 {
-    return ((int64_t)z * L + y) * L + (x & 1) * (L >> 1) + (x >> 1);
+    const int64_t half = L >> 1, nq = (int64_t)L * L * H / 4;
+    const int c = (x & 1) | (((y + z) & 1) << 1);
+    return c * nq + ((int64_t)z * half + (y >> 1)) * half + (x >> 1);
+}
+
+// Inverse of internal_index.
+__host__ __device__ inline void site_of_internal(int32_t L, int32_t H, int64_t i, int &x, int &y, int &z)
+{
+    const int64_t half = L >> 1, nq = (int64_t)L * L * H / 4;
+    const int c = (int)(i / nq);
+    const int64_t q = i - c * nq;
+    const int64_t u = q / half;
+    z = (int)(u / half);
+    const int yh = (int)(u - (int64_t)z * half);
+    x = 2 * (int)(q - u * half) + (c & 1);
+    y = 2 * yh + (((c >> 1) ^ z) & 1);
 }
 
 // ---------------------------------------------------------------------------

@@ -153,7 +153,7 @@ void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p,
         for (int y = 0; y < L; ++y)
             for (int x = 0; x < L; ++x) {
                 const int i = ref_site(L, x, y, z);
-                const int64_t ii = internal_index(L, x, y, z);
+                const int64_t ii = internal_index(L, H, x, y, z);
                 const float kxp = Kx[i], kxm = Kx[ref_site(L, (x + L - 1) % L, y, z)];
                 const float kyp = Ky[i], kym = Ky[ref_site(L, x, (y + L - 1) % L, z)];
                 const double sK = (((double)kxp + (double)kxm) + (double)kyp) + (double)kym;
@@ -437,7 +437,7 @@ void to_internal(int L, int H, const int8_t *ref, uint8_t *out)
 {
     for (int z = 0; z < H; ++z)
         for (int y = 0; y < L; ++y)
-            for (int x = 0; x < L; ++x) out[internal_index(L, x, y, z)] = ref[ref_site(L, x, y, z)] > 0 ? 1 : 0;
+            for (int x = 0; x < L; ++x) out[internal_index(L, H, x, y, z)] = ref[ref_site(L, x, y, z)] > 0 ? 1 : 0;
 }
 
 template <typename S, typename D>
@@ -445,7 +445,7 @@ void from_internal(int L, int H, const S *in, D *ref, D (*conv)(S))
 {
     for (int z = 0; z < H; ++z)
         for (int y = 0; y < L; ++y)
-            for (int x = 0; x < L; ++x) ref[ref_site(L, x, y, z)] = conv(in[internal_index(L, x, y, z)]);
+            for (int x = 0; x < L; ++x) ref[ref_site(L, x, y, z)] = conv(in[internal_index(L, H, x, y, z)]);
 }
 
 int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **handle)

@@ -92,79 +92,72 @@ __device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N])
             for (int k = 0; k < OBS_N; ++k) v[k] += red[w][k];
 }
 
-// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) over the dwords
-// [w0, N/4) step `stride` of a lattice held in the internal layout (L % 8 == 0).
-// l_i = 2 (a(+y) + a(-y) - a(+x) - a(-x)), a = 1 if the bond is anti-aligned.
-__device__ __forceinline__ void locnem_dwords(const uint32_t *sp32, int L, int H, int w0, int stride,
-                                              int32_t *locspin, int32_t *locnem)
+// ---------------------------------------------------------------------------
+// Half-row units of the colour-plane layout (L % 8 == 0, WPH = L/8 dwords per
+// half-row).  Plane c holds N/4 bytes = NQ dwords; unit u = z*L/2 + y/2 of
+// plane c sits at dwords [c*NQ + u*WPH, +WPH).  Its neighbour half-rows:
+//   +-x and the diagonals' x part : plane c^1 (same row), plane c^3 (rows y+-1)
+//   +-y                           : plane c^2, units uyp / uym
+//   +-z                           : plane c^2, units u +- L/2 (periodic in z)
+// Consecutive threads take consecutive units, so every neighbour read of a
+// wave is one contiguous run of LDS (or global) memory.
+// ---------------------------------------------------------------------------
+struct UnitGeo {
+    int z, p;                 // p = y & 1
+    int uyp, uym, uzp, uzm;
+};
+
+template <int WPH>
+__device__ __forceinline__ UnitGeo unit_geo(int u, int c, int H)
 {
-    const int wph = L >> 3, dpr = L >> 2, N = L * L * H;
-    for (int w = w0; w < N / 4; w += stride) {
-        const int row = w / dpr, within = w - row * dpr;
-        const int hx = within >= wph ? 1 : 0, kk = within - hx * wph;
-        const int y = row % L, z = row / L;
-        const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
-        const int oh = (1 - hx) * wph;
-        const int kb = hx ? ((kk == wph - 1) ? 0 : kk + 1) : ((kk == 0) ? wph - 1 : kk - 1);
-        const uint32_t S = sp32[w];
-        const uint32_t YP = sp32[(z * L + yp) * dpr + hx * wph + kk], YM = sp32[(z * L + ym) * dpr + hx * wph + kk];
-        const uint32_t A0 = sp32[row * dpr + oh + kk], B0 = sp32[row * dpr + oh + kb];
-        const uint32_t Xp = hx ? __builtin_amdgcn_alignbyte(B0, A0, 1) : A0;
-        const uint32_t Xm = hx ? A0 : __builtin_amdgcn_alignbyte(A0, B0, 3);
-        const uint32_t ax = (S ^ Xp) + (S ^ Xm), ay = (S ^ YP) + (S ^ YM);
-        int4 ls = reinterpret_cast<int4 *>(locspin)[w], ln = reinterpret_cast<int4 *>(locnem)[w];
-        int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&ln);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            lsp[j] += (int)((S >> (8 * j)) & 1u) * 2 - 1;
-            lnp[j] += 2 * ((int)((ay >> (8 * j)) & 0xffu) - (int)((ax >> (8 * j)) & 0xffu));
-        }
-        reinterpret_cast<int4 *>(locspin)[w] = ls;
-        reinterpret_cast<int4 *>(locnem)[w] = ln;
+    constexpr int HALF = 4 * WPH;
+    UnitGeo g;
+    g.z = u / HALF;
+    const int yh = u - g.z * HALF;
+    g.p = ((c >> 1) ^ g.z) & 1;
+    if (g.p) {                                  // y odd: y+1 is the next unit, y-1 this one
+        g.uyp = (yh == HALF - 1) ? u - (HALF - 1) : u + 1;
+        g.uym = u;
+    } else {                                    // y even: y+1 is this unit, y-1 the previous one
+        g.uyp = u;
+        g.uym = (yh == 0) ? u + (HALF - 1) : u - 1;
     }
+    g.uzp = (g.z == H - 1) ? u - (H - 1) * HALF : u + HALF;
+    g.uzm = (g.z == 0) ? u + (H - 1) * HALF : u - HALF;
+    return g;
 }
 
-// ---------------------------------------------------------------------------
-// LDS-resident FP32 sweep.  One block per lattice; the lattice's spins (one
-// byte per site, internal layout) live in LDS for all PTSWAPEVER sweeps of the
-// cycle, and the per-site FP32 constants stream from L2/HBM as float4 (one per
-// plane per 4 sites).  In a colour pass a thread owns one half-row: the L/2
-// same-colour sites (x of one parity) of a row (y, z), WPH = L/8 LDS dwords.
-// Its neighbours are whole half-rows too: +-y and +-z are the same half of the
-// adjacent rows; +-x and the four diagonals are the other half of rows y,
-// y+1, y-1 shifted by one byte (v_alignbyte, with wrap inside the half-row).
-// ---------------------------------------------------------------------------
 template <int WPH>
-__device__ __forceinline__ void lds_load(const uint32_t *lds32, int base, uint32_t (&v)[WPH])
+__device__ __forceinline__ void load_words(const uint32_t *base, int off, uint32_t (&v)[WPH])
 {
     if constexpr (WPH == 4) {
-        const uint4 q = *reinterpret_cast<const uint4 *>(lds32 + base);
+        const uint4 q = *reinterpret_cast<const uint4 *>(base + off);
         v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
     } else if constexpr (WPH == 2) {
-        const uint2 q = *reinterpret_cast<const uint2 *>(lds32 + base);
+        const uint2 q = *reinterpret_cast<const uint2 *>(base + off);
         v[0] = q.x; v[1] = q.y;
     } else {
 #pragma unroll
-        for (int w = 0; w < WPH; ++w) v[w] = lds32[base + w];
+        for (int w = 0; w < WPH; ++w) v[w] = base[off + w];
     }
 }
 
 template <int WPH>
-__device__ __forceinline__ void lds_store(uint32_t *lds32, int base, const uint32_t (&v)[WPH])
+__device__ __forceinline__ void store_words(uint32_t *base, int off, const uint32_t (&v)[WPH])
 {
     if constexpr (WPH == 4) {
-        *reinterpret_cast<uint4 *>(lds32 + base) = uint4{v[0], v[1], v[2], v[3]};
+        *reinterpret_cast<uint4 *>(base + off) = uint4{v[0], v[1], v[2], v[3]};
     } else if constexpr (WPH == 2) {
-        *reinterpret_cast<uint2 *>(lds32 + base) = uint2{v[0], v[1]};
+        *reinterpret_cast<uint2 *>(base + off) = uint2{v[0], v[1]};
     } else {
 #pragma unroll
-        for (int w = 0; w < WPH; ++w) lds32[base + w] = v[w];
+        for (int w = 0; w < WPH; ++w) base[off + w] = v[w];
     }
 }
 
-// +x / -x neighbour bytes of a half-row from the other half (A) of a row:
-// even sites (HX = 0, x = 2m): +x = odd m, -x = odd m-1;
-// odd sites  (HX = 1, x = 2m+1): -x = even m, +x = even m+1.
+// +x / -x neighbour bytes of a half-row from the other-parity half-row A of a
+// row: even sites (HX = 0, x = 2m): +x = odd m, -x = odd m-1;
+// odd sites  (HX = 1, x = 2m+1): -x = even m, +x = even m+1 (periodic).
 template <int WPH, int HX>
 __device__ __forceinline__ void x_neighbours(const uint32_t (&A)[WPH], uint32_t (&Xp)[WPH], uint32_t (&Xm)[WPH])
 {
@@ -180,6 +173,97 @@ __device__ __forceinline__ void x_neighbours(const uint32_t (&A)[WPH], uint32_t 
     }
 }
 
+// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) over the units of
+// one lattice in the colour-plane layout (sp32 in LDS or global memory):
+// l_i = 2 (a(+y) + a(-y) - a(+x) - a(-x)), a = 1 if the bond is anti-aligned.
+template <int WPH, int HX>
+__device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
+                                             int32_t *locspin, int32_t *locnem)
+{
+    constexpr int HALF = 4 * WPH;
+    const int nq = 4 * WPH * WPH * H, nunits = HALF * H;
+    for (int u = t0; u < nunits; u += stride) {
+        const UnitGeo g = unit_geo<WPH>(u, c, H);
+        uint32_t S[WPH], A[WPH], Xp[WPH], Xm[WPH], YP[WPH], YM[WPH];
+        load_words<WPH>(sp32, c * nq + u * WPH, S);
+        load_words<WPH>(sp32, (c ^ 1) * nq + u * WPH, A);
+        x_neighbours<WPH, HX>(A, Xp, Xm);
+        load_words<WPH>(sp32, (c ^ 2) * nq + g.uyp * WPH, YP);
+        load_words<WPH>(sp32, (c ^ 2) * nq + g.uym * WPH, YM);
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) {
+            const int dw = c * nq + u * WPH + w;
+            const uint32_t ax = (S[w] ^ Xp[w]) + (S[w] ^ Xm[w]), ay = (S[w] ^ YP[w]) + (S[w] ^ YM[w]);
+            int4 ls = reinterpret_cast<int4 *>(locspin)[dw], ln = reinterpret_cast<int4 *>(locnem)[dw];
+            int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&ln);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                lsp[j] += (int)((S[w] >> (8 * j)) & 1u) * 2 - 1;
+                lnp[j] += 2 * ((int)((ay >> (8 * j)) & 0xffu) - (int)((ax >> (8 * j)) & 0xffu));
+            }
+            reinterpret_cast<int4 *>(locspin)[dw] = ls;
+            reinterpret_cast<int4 *>(locnem)[dw] = ln;
+        }
+    }
+}
+
+template <int WPH>
+__device__ __forceinline__ void locnem_units(const uint32_t *sp32, int H, int t0, int stride, int32_t *locspin,
+                                             int32_t *locnem)
+{
+    for (int c = 0; c < 4; ++c) {
+        if (c & 1) locnem_plane<WPH, 1>(sp32, c, H, t0, stride, locspin, locnem);
+        else locnem_plane<WPH, 0>(sp32, c, H, t0, stride, locspin, locnem);
+    }
+}
+
+// Integer observables (see k_observables) of one colour plane, forward bonds.
+template <int WPH, int HX>
+__device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
+                                              const int64_t *qb, int64_t (&v)[OBS_N])
+{
+    constexpr int HALF = 4 * WPH;
+    const int nq = 4 * WPH * WPH * H, nunits = HALF * H;
+    for (int u = t0; u < nunits; u += stride) {
+        const UnitGeo g = unit_geo<WPH>(u, c, H);
+        uint32_t S[WPH], A[WPH], Xp0[WPH], Xm0[WPH], Xp1[WPH], Xm1[WPH], YP[WPH], YM[WPH], ZP[WPH];
+        load_words<WPH>(sp32, c * nq + u * WPH, S);
+        load_words<WPH>(sp32, (c ^ 1) * nq + u * WPH, A);
+        x_neighbours<WPH, HX>(A, Xp0, Xm0);
+        load_words<WPH>(sp32, (c ^ 3) * nq + g.uyp * WPH, A);
+        x_neighbours<WPH, HX>(A, Xp1, Xm1);
+        load_words<WPH>(sp32, (c ^ 2) * nq + g.uyp * WPH, YP);
+        load_words<WPH>(sp32, (c ^ 2) * nq + g.uym * WPH, YM);
+        load_words<WPH>(sp32, (c ^ 2) * nq + g.uzp * WPH, ZP);
+#pragma unroll
+        for (int w = 0; w < WPH; ++w) {
+            const uint32_t Sw = S[w];
+            const int m4 = 2 * __builtin_popcount(Sw) - 4;
+            v[OBS_M] += m4;
+            v[OBS_SX] += HX ? -m4 : m4;
+            v[OBS_SY] += g.p ? -m4 : m4;
+            v[OBS_BX] += 4 - 2 * __builtin_popcount(Sw ^ Xp0[w]);
+            v[OBS_BY] += 4 - 2 * __builtin_popcount(Sw ^ YP[w]);
+            v[OBS_BZ] += 4 - 2 * __builtin_popcount(Sw ^ ZP[w]);
+            v[OBS_BD] += 8 - 2 * (__builtin_popcount(Sw ^ Xp1[w]) + __builtin_popcount(Sw ^ Xm1[w]));
+            const uint32_t ax = (Sw ^ Xp0[w]) + (Sw ^ Xm0[w]), ay = (Sw ^ YP[w]) + (Sw ^ YM[w]);
+            const int64_t *qw = qb + 4 * (int64_t)(c * nq + u * WPH + w);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int l = 2 * ((int)((ay >> (8 * j)) & 0xff) - (int)((ax >> (8 * j)) & 0xff));
+                v[OBS_EQ] += qw[j] * (int64_t)l;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-resident FP32 sweep.  One block per lattice; the lattice's spins live in
+// LDS for all PTSWAPEVER sweeps of the cycle, and the per-site FP32 constants
+// stream from L2/HBM as float4 (one per plane per 4 sites).  In a colour pass
+// each thread owns units of the colour's plane; lane i of a wave reads the
+// 4*WPH bytes at unit u0+i of each neighbour plane.
+// ---------------------------------------------------------------------------
 struct SweepCtx {
     const float4 *kc4, *kxp4, *kxm4, *kyp4, *kym4;
     const double *p64;     // config's FP64 field (shadow evaluation)
@@ -189,45 +273,42 @@ struct SweepCtx {
 };
 
 template <int WPH, int HX, bool SHADOW>
-__device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc,
-                                                int u, int z, int y, int H, unsigned long long &sh_prop,
-                                                unsigned long long &sh_mis)
+__device__ __forceinline__ void update_unit(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c,
+                                            int u, int H, unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
-    constexpr int L = 8 * WPH, DPR = 2 * WPH, HALF = 4 * WPH;
-    constexpr int OWN = HX * WPH, OTH = (1 - HX) * WPH;
-    const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
-    const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
-    const int R0 = (z * L + y) * DPR, RYp = (z * L + yp) * DPR, RYm = (z * L + ym) * DPR;
-    const int RZp = (zp * L + y) * DPR, RZm = (zm * L + y) * DPR;
+    constexpr int HALF = 4 * WPH;
+    const int nq = 4 * WPH * WPH * H;
+    const UnitGeo g = unit_geo<WPH>(u, c, H);
+    const int own = c * nq + u * WPH, px = (c ^ 1) * nq, py = (c ^ 2) * nq, pd = (c ^ 3) * nq;
     // Only the reduced neighbour words stay live: Z = b(+z) + b(-z) and
     // D = sum of the four diagonal b's, bytewise (<= 2 and <= 4: no carries).
     uint32_t S[WPH], YP[WPH], YM[WPH], Z[WPH], D[WPH], Xp0[WPH], Xm0[WPH];
     {
         uint32_t T[WPH];
-        lds_load<WPH>(lds32, RZp + OWN, Z);
-        lds_load<WPH>(lds32, RZm + OWN, T);
+        load_words<WPH>(lds32, py + g.uzp * WPH, Z);
+        load_words<WPH>(lds32, py + g.uzm * WPH, T);
 #pragma unroll
         for (int w = 0; w < WPH; ++w) Z[w] += T[w];
     }
     {
         uint32_t A[WPH], P[WPH], M[WPH];
-        lds_load<WPH>(lds32, RYp + OTH, A);
+        load_words<WPH>(lds32, pd + g.uyp * WPH, A);
         x_neighbours<WPH, HX>(A, P, M);
 #pragma unroll
         for (int w = 0; w < WPH; ++w) D[w] = P[w] + M[w];
-        lds_load<WPH>(lds32, RYm + OTH, A);
+        load_words<WPH>(lds32, pd + g.uym * WPH, A);
         x_neighbours<WPH, HX>(A, P, M);
 #pragma unroll
         for (int w = 0; w < WPH; ++w) D[w] += P[w] + M[w];
-        lds_load<WPH>(lds32, R0 + OTH, A);
+        load_words<WPH>(lds32, px + u * WPH, A);
         x_neighbours<WPH, HX>(A, Xp0, Xm0);
     }
-    lds_load<WPH>(lds32, RYp + OWN, YP);
-    lds_load<WPH>(lds32, RYm + OWN, YM);
-    lds_load<WPH>(lds32, R0 + OWN, S);
+    load_words<WPH>(lds32, py + g.uyp * WPH, YP);
+    load_words<WPH>(lds32, py + g.uym * WPH, YM);
+    load_words<WPH>(lds32, own, S);
 #pragma unroll
     for (int w = 0; w < WPH; ++w) {
-        const int w4 = R0 + OWN + w;                      // float4 index of these 4 sites
+        const int w4 = own + w;                           // float4 index of these 4 sites
         const float4 kc = cx.kc4[w4], kxp = cx.kxp4[w4], kxm = cx.kxm4[w4], kyp = cx.kyp4[w4],
                      kym = cx.kym4[w4];
         const U4 rn = philox4x32_10((uint32_t)(u * WPH + w), cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
@@ -253,12 +334,12 @@ __device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx 
                 // FP64 rule on the same state and draw; the individual
                 // neighbour words are re-read (no colour-c site changes them)
                 uint32_t A1[WPH], A2[WPH], P1[WPH], M1[WPH], P2[WPH], M2[WPH], ZP[WPH], ZM[WPH];
-                lds_load<WPH>(lds32, RYp + OTH, A1);
-                lds_load<WPH>(lds32, RYm + OTH, A2);
+                load_words<WPH>(lds32, pd + g.uyp * WPH, A1);
+                load_words<WPH>(lds32, pd + g.uym * WPH, A2);
                 x_neighbours<WPH, HX>(A1, P1, M1);
                 x_neighbours<WPH, HX>(A2, P2, M2);
-                lds_load<WPH>(lds32, RZp + OWN, ZP);
-                lds_load<WPH>(lds32, RZm + OWN, ZM);
+                load_words<WPH>(lds32, py + g.uzp * WPH, ZP);
+                load_words<WPH>(lds32, py + g.uzm * WPH, ZM);
                 const int m = 4 * w + j;                          // position in the half-row
                 const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
                 const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
@@ -267,9 +348,9 @@ __device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx 
                                        pm1(byte_bit(P1[w], j)), pm1(byte_bit(M1[w], j)),
                                        pm1(byte_bit(P2[w], j)), pm1(byte_bit(M2[w], j)),
                                        pm1(byte_bit(ZP[w], j)), pm1(byte_bit(ZM[w], j))};
-                const double pv[5] = {cx.p64[4 * (R0 + OWN) + m], cx.p64[4 * (RYp + OWN) + m],
-                                      cx.p64[4 * (RYm + OWN) + m], cx.p64[4 * (R0 + OTH) + mo_p],
-                                      cx.p64[4 * (R0 + OTH) + mo_m]};
+                const double pv[5] = {cx.p64[4 * own + m], cx.p64[4 * (py + g.uyp * WPH) + m],
+                                      cx.p64[4 * (py + g.uym * WPH) + m], cx.p64[4 * (px + u * WPH) + mo_p],
+                                      cx.p64[4 * (px + u * WPH) + mo_m]};
                 const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[j]);
                 sh_prop += 1;
                 sh_mis += (a64 != acc);
@@ -278,17 +359,17 @@ __device__ __forceinline__ void update_half_row(uint32_t *lds32, const SweepCtx 
         }
         S[w] ^= flip;
     }
-    lds_store<WPH>(lds32, R0 + OWN, S);
+    store_words<WPH>(lds32, own, S);
 }
 
 template <int WPH, int NT, bool SHADOW>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
                                               uint32_t *lds32, int64_t (*red)[OBS_N])
 {
-    constexpr int L = 8 * WPH, HALF = 4 * WPH;
+    constexpr int HALF = 4 * WPH;
     const int H = d.H;
     const int tid = threadIdx.x;
-    const int N = L * L * H;
+    const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
     for (int i = tid; i < N / 16; i += NT)
@@ -317,7 +398,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     // the previous cycle measured and left this replica at slot PTMEASINDEX:
     // accumulate its local nematic sums from the state it ended in
     if (st.fuse_locnem && slot == st.ptmeasindex - 1 && st.locnem_pending[d.conf]) {
-        locnem_dwords(lds32, L, H, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
+        locnem_units<WPH>(lds32, H, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
         if (tid == 0) st.locsamples[d.conf] += 1;
         __syncthreads();   // every read of the unswept state precedes the first colour pass
     }
@@ -328,13 +409,10 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         cx.sweep_hi = (uint32_t)(sweep >> 32);
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
-            const int pr = c >> 1;
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
             for (int u = tid; u < nunits; u += NT) {
-                const int z = u / HALF, yh = u - z * HALF;   // u = z*L/2 + y/2
-                const int y = 2 * yh + ((pr ^ z) & 1);
-                if (c & 1) update_half_row<WPH, 1, SHADOW>(lds32, cx, cc, u, z, y, H, sh_prop, sh_mis);
-                else update_half_row<WPH, 0, SHADOW>(lds32, cx, cc, u, z, y, H, sh_prop, sh_mis);
+                if (c & 1) update_unit<WPH, 1, SHADOW>(lds32, cx, cc, c, u, H, sh_prop, sh_mis);
+                else update_unit<WPH, 0, SHADOW>(lds32, cx, cc, c, u, H, sh_prop, sh_mis);
             }
             __syncthreads();
         }
@@ -342,44 +420,10 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 
     // ---- observables of the final state (integer sums; phi part fixed point)
     int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-    constexpr int wph = WPH, dpr = 2 * WPH, half = HALF;
     const int64_t *qb = st.q + d.site_off;
-    for (int w = tid; w < N / 4; w += NT) {
-        const int row = w / dpr, within = w - row * dpr;
-        const int hx = within >= wph ? 1 : 0, kk = within - hx * wph;
-        const int y = row % L, z = row / L;
-        const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
-        const int zp = (z == H - 1) ? 0 : z + 1;
-        const int r0 = row * L, rYp = (z * L + yp) * L, rYm = (z * L + ym) * L, rZp = (zp * L + y) * L;
-        const int oh = (1 - hx) * half / 4;
-        const int kb = hx ? ((kk == wph - 1) ? 0 : kk + 1) : ((kk == 0) ? wph - 1 : kk - 1);
-        const uint32_t S = lds32[w];
-        const uint32_t YP = lds32[(rYp + hx * half) / 4 + kk], YM = lds32[(rYm + hx * half) / 4 + kk];
-        const uint32_t ZP = lds32[(rZp + hx * half) / 4 + kk];
-        const uint32_t A0 = lds32[r0 / 4 + oh + kk], B0 = lds32[r0 / 4 + oh + kb];
-        const uint32_t A1 = lds32[rYp / 4 + oh + kk], B1 = lds32[rYp / 4 + oh + kb];
-        uint32_t Xp0, Xm0, Xp1, Xm1;
-        if (hx == 0) {
-            Xp0 = A0; Xm0 = __builtin_amdgcn_alignbyte(A0, B0, 3);
-            Xp1 = A1; Xm1 = __builtin_amdgcn_alignbyte(A1, B1, 3);
-        } else {
-            Xm0 = A0; Xp0 = __builtin_amdgcn_alignbyte(B0, A0, 1);
-            Xm1 = A1; Xp1 = __builtin_amdgcn_alignbyte(B1, A1, 1);
-        }
-        const int m4 = 2 * __builtin_popcount(S) - 4;
-        v[OBS_M] += m4;
-        v[OBS_SX] += hx ? -m4 : m4;
-        v[OBS_SY] += (y & 1) ? -m4 : m4;
-        v[OBS_BX] += 4 - 2 * __builtin_popcount(S ^ Xp0);
-        v[OBS_BY] += 4 - 2 * __builtin_popcount(S ^ YP);
-        v[OBS_BZ] += 4 - 2 * __builtin_popcount(S ^ ZP);
-        v[OBS_BD] += 8 - 2 * (__builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1));
-        const uint32_t ax = (S ^ Xp0) + (S ^ Xm0), ay = (S ^ YP) + (S ^ YM);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int l = 2 * ((int)((ay >> (8 * j)) & 0xff) - (int)((ax >> (8 * j)) & 0xff));
-            v[OBS_EQ] += qb[4 * w + j] * (int64_t)l;
-        }
+    for (int c = 0; c < 4; ++c) {
+        if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, qb, v);
+        else observe_plane<WPH, 0>(lds32, c, H, tid, NT, qb, v);
     }
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
@@ -465,12 +509,12 @@ __device__ __forceinline__ Nbr neighbours(int L, int H, int x, int y, int z)
     const int yp = (y == L - 1) ? 0 : y + 1, ym = (y == 0) ? L - 1 : y - 1;
     const int zp = (z == H - 1) ? 0 : z + 1, zm = (z == 0) ? H - 1 : z - 1;
     Nbr n;
-    n.i = internal_index(L, x, y, z);
-    n.m[0] = internal_index(L, x, yp, z);  n.m[1] = internal_index(L, x, ym, z);
-    n.m[2] = internal_index(L, xp, y, z);  n.m[3] = internal_index(L, xm, y, z);
-    n.m[4] = internal_index(L, xp, yp, z); n.m[5] = internal_index(L, xm, yp, z);
-    n.m[6] = internal_index(L, xp, ym, z); n.m[7] = internal_index(L, xm, ym, z);
-    n.m[8] = internal_index(L, x, y, zp);  n.m[9] = internal_index(L, x, y, zm);
+    n.i = internal_index(L, H, x, y, z);
+    n.m[0] = internal_index(L, H, x, yp, z);  n.m[1] = internal_index(L, H, x, ym, z);
+    n.m[2] = internal_index(L, H, xp, y, z);  n.m[3] = internal_index(L, H, xm, y, z);
+    n.m[4] = internal_index(L, H, xp, yp, z); n.m[5] = internal_index(L, H, xm, yp, z);
+    n.m[6] = internal_index(L, H, xp, ym, z); n.m[7] = internal_index(L, H, xm, ym, z);
+    n.m[8] = internal_index(L, H, x, y, zp);  n.m[9] = internal_index(L, H, x, y, zm);
     return n;
 }
 
@@ -556,14 +600,13 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
     __shared__ int64_t red[OBS_THREADS / 64][OBS_N];
     const int lat = lat_begin + blockIdx.x;
     const LatDesc d = st.lats[lat];
-    const int L = d.L, H = d.H, N = d.N, half = L >> 1;
+    const int L = d.L, H = d.H, N = d.N;
     const uint8_t *sp = st.spins + d.spin_off;
     const int64_t *qb = st.q + d.site_off;
     int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < N; i += OBS_THREADS) {
-        const int row = i / L, within = i - row * L;
-        const int hx = within >= half ? 1 : 0;
-        const int x = 2 * (within - hx * half) + hx, y = row % L, z = row / L;
+        int x, y, z;
+        site_of_internal(L, H, i, x, y, z);
         const Nbr nb = neighbours(L, H, x, y, z);
         const int si = sp[i] ? 1 : -1;
         int s[10];
@@ -719,24 +762,29 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
     const ConfDesc cd = st.confs[conf];
     const int r = st.rep_at_temp[cd.slot_base + st.ptmeasindex - 1];
     const LatDesc d = st.lats[cd.lat0 + r];
-    const int L = d.L, H = d.H, N = d.N, half = L >> 1;
+    const int L = d.L, H = d.H, N = d.N;
     const uint8_t *sp = st.spins + d.spin_off;
+    const uint32_t *sp32 = reinterpret_cast<const uint32_t *>(sp);
+    int32_t *ls = st.locspin + d.site_off, *ln = st.locnem + d.site_off;
     const int t0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
-    if ((L & 7) == 0) {
-        locnem_dwords(reinterpret_cast<const uint32_t *>(sp), L, H, t0, stride, st.locspin + d.site_off,
-                      st.locnem + d.site_off);
-    } else {
-        for (int i = t0; i < N; i += stride) {
-            const int row = i / L, within = i - row * L;
-            const int hx = within >= half ? 1 : 0;
-            const int x = 2 * (within - hx * half) + hx, y = row % L, z = row / L;
-            const Nbr nb = neighbours(L, H, x, y, z);
-            const int si = sp[i] ? 1 : -1;
-            const int l = si * ((sp[nb.m[2]] ? 1 : -1) - (sp[nb.m[0]] ? 1 : -1) + (sp[nb.m[3]] ? 1 : -1) -
-                                (sp[nb.m[1]] ? 1 : -1));
-            st.locspin[d.site_off + i] += si;
-            st.locnem[d.site_off + i] += l;
-        }
+    switch (L) {
+        case 8: locnem_units<1>(sp32, H, t0, stride, ls, ln); break;
+        case 16: locnem_units<2>(sp32, H, t0, stride, ls, ln); break;
+        case 24: locnem_units<3>(sp32, H, t0, stride, ls, ln); break;
+        case 32: locnem_units<4>(sp32, H, t0, stride, ls, ln); break;
+        case 40: locnem_units<5>(sp32, H, t0, stride, ls, ln); break;
+        case 48: locnem_units<6>(sp32, H, t0, stride, ls, ln); break;
+        default:
+            for (int i = t0; i < N; i += stride) {
+                int x, y, z;
+                site_of_internal(L, H, i, x, y, z);
+                const Nbr nb = neighbours(L, H, x, y, z);
+                const int si = sp[i] ? 1 : -1;
+                const int l = si * ((sp[nb.m[2]] ? 1 : -1) - (sp[nb.m[0]] ? 1 : -1) + (sp[nb.m[3]] ? 1 : -1) -
+                                    (sp[nb.m[1]] ? 1 : -1));
+                ls[i] += si;
+                ln[i] += l;
+            }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) st.locsamples[conf] += 1;
 }
