@@ -127,49 +127,20 @@ __device__ __forceinline__ UnitGeo unit_geo(int u, int c, int H)
     return g;
 }
 
-template <int WPH>
-__device__ __forceinline__ void load_words(const uint32_t *base, int off, uint32_t (&v)[WPH])
-{
-    if constexpr (WPH == 4) {
-        const uint4 q = *reinterpret_cast<const uint4 *>(base + off);
-        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
-    } else if constexpr (WPH == 2) {
-        const uint2 q = *reinterpret_cast<const uint2 *>(base + off);
-        v[0] = q.x; v[1] = q.y;
-    } else {
-#pragma unroll
-        for (int w = 0; w < WPH; ++w) v[w] = base[off + w];
-    }
-}
-
-template <int WPH>
-__device__ __forceinline__ void store_words(uint32_t *base, int off, const uint32_t (&v)[WPH])
-{
-    if constexpr (WPH == 4) {
-        *reinterpret_cast<uint4 *>(base + off) = uint4{v[0], v[1], v[2], v[3]};
-    } else if constexpr (WPH == 2) {
-        *reinterpret_cast<uint2 *>(base + off) = uint2{v[0], v[1]};
-    } else {
-#pragma unroll
-        for (int w = 0; w < WPH; ++w) base[off + w] = v[w];
-    }
-}
-
-// +x / -x neighbour bytes of a half-row from the other-parity half-row A of a
-// row: even sites (HX = 0, x = 2m): +x = odd m, -x = odd m-1;
-// odd sites  (HX = 1, x = 2m+1): -x = even m, +x = even m+1 (periodic).
+// +x / -x neighbour bytes of dword w of the half-row at dword offset `uoff`,
+// taken from the other-parity half-row of the same row (periodic in x).
 template <int WPH, int HX>
-__device__ __forceinline__ void x_neighbours(const uint32_t (&A)[WPH], uint32_t (&Xp)[WPH], uint32_t (&Xm)[WPH])
+__device__ __forceinline__ void x_pair(const uint32_t *base, int uoff, int w, uint32_t &Xp, uint32_t &Xm)
 {
-#pragma unroll
-    for (int w = 0; w < WPH; ++w) {
-        if constexpr (HX == 0) {
-            Xp[w] = A[w];
-            Xm[w] = __builtin_amdgcn_alignbyte(A[w], A[(w + WPH - 1) % WPH], 3);
-        } else {
-            Xm[w] = A[w];
-            Xp[w] = __builtin_amdgcn_alignbyte(A[(w + 1) % WPH], A[w], 1);
-        }
+    const uint32_t A = base[uoff + w];
+    if constexpr (HX == 0) {
+        const uint32_t B = base[uoff + (w == 0 ? WPH - 1 : w - 1)];
+        Xp = A;
+        Xm = __builtin_amdgcn_alignbyte(A, B, 3);
+    } else {
+        const uint32_t B = base[uoff + (w == WPH - 1 ? 0 : w + 1)];
+        Xm = A;
+        Xp = __builtin_amdgcn_alignbyte(B, A, 1);
     }
 }
 
@@ -180,30 +151,25 @@ template <int WPH, int HX>
 __device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
                                              int32_t *locspin, int32_t *locnem)
 {
-    constexpr int HALF = 4 * WPH;
-    const int nq = 4 * WPH * WPH * H, nunits = HALF * H;
-    for (int u = t0; u < nunits; u += stride) {
+    const int nq = 4 * WPH * WPH * H;
+    for (int t = t0; t < nq; t += stride) {
+        const int u = t / WPH, w = t - u * WPH;
         const UnitGeo g = unit_geo<WPH>(u, c, H);
-        uint32_t S[WPH], A[WPH], Xp[WPH], Xm[WPH], YP[WPH], YM[WPH];
-        load_words<WPH>(sp32, c * nq + u * WPH, S);
-        load_words<WPH>(sp32, (c ^ 1) * nq + u * WPH, A);
-        x_neighbours<WPH, HX>(A, Xp, Xm);
-        load_words<WPH>(sp32, (c ^ 2) * nq + g.uyp * WPH, YP);
-        load_words<WPH>(sp32, (c ^ 2) * nq + g.uym * WPH, YM);
+        const int py = (c ^ 2) * nq, dw = c * nq + t;
+        const uint32_t S = sp32[dw];
+        const uint32_t YP = sp32[py + g.uyp * WPH + w], YM = sp32[py + g.uym * WPH + w];
+        uint32_t Xp, Xm;
+        x_pair<WPH, HX>(sp32, (c ^ 1) * nq + u * WPH, w, Xp, Xm);
+        const uint32_t ax = (S ^ Xp) + (S ^ Xm), ay = (S ^ YP) + (S ^ YM);
+        int4 ls = reinterpret_cast<int4 *>(locspin)[dw], lv = reinterpret_cast<int4 *>(locnem)[dw];
+        int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
 #pragma unroll
-        for (int w = 0; w < WPH; ++w) {
-            const int dw = c * nq + u * WPH + w;
-            const uint32_t ax = (S[w] ^ Xp[w]) + (S[w] ^ Xm[w]), ay = (S[w] ^ YP[w]) + (S[w] ^ YM[w]);
-            int4 ls = reinterpret_cast<int4 *>(locspin)[dw], ln = reinterpret_cast<int4 *>(locnem)[dw];
-            int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&ln);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                lsp[j] += (int)((S[w] >> (8 * j)) & 1u) * 2 - 1;
-                lnp[j] += 2 * ((int)((ay >> (8 * j)) & 0xffu) - (int)((ax >> (8 * j)) & 0xffu));
-            }
-            reinterpret_cast<int4 *>(locspin)[dw] = ls;
-            reinterpret_cast<int4 *>(locnem)[dw] = ln;
+        for (int j = 0; j < 4; ++j) {
+            lsp[j] += (int)((S >> (8 * j)) & 1u) * 2 - 1;
+            lnp[j] += 2 * ((int)((ay >> (8 * j)) & 0xffu) - (int)((ax >> (8 * j)) & 0xffu));
         }
+        reinterpret_cast<int4 *>(locspin)[dw] = ls;
+        reinterpret_cast<int4 *>(locnem)[dw] = lv;
     }
 }
 
@@ -217,42 +183,37 @@ __device__ __forceinline__ void locnem_units(const uint32_t *sp32, int H, int t0
     }
 }
 
-// Integer observables (see k_observables) of one colour plane, forward bonds.
+// Integer observables (see k_observables) of one colour plane, forward bonds,
+// one dword (4 sites) per step.
 template <int WPH, int HX>
 __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
                                               const int64_t *qb, int64_t (&v)[OBS_N])
 {
-    constexpr int HALF = 4 * WPH;
-    const int nq = 4 * WPH * WPH * H, nunits = HALF * H;
-    for (int u = t0; u < nunits; u += stride) {
+    const int nq = 4 * WPH * WPH * H;
+    for (int t = t0; t < nq; t += stride) {
+        const int u = t / WPH, w = t - u * WPH;
         const UnitGeo g = unit_geo<WPH>(u, c, H);
-        uint32_t S[WPH], A[WPH], Xp0[WPH], Xm0[WPH], Xp1[WPH], Xm1[WPH], YP[WPH], YM[WPH], ZP[WPH];
-        load_words<WPH>(sp32, c * nq + u * WPH, S);
-        load_words<WPH>(sp32, (c ^ 1) * nq + u * WPH, A);
-        x_neighbours<WPH, HX>(A, Xp0, Xm0);
-        load_words<WPH>(sp32, (c ^ 3) * nq + g.uyp * WPH, A);
-        x_neighbours<WPH, HX>(A, Xp1, Xm1);
-        load_words<WPH>(sp32, (c ^ 2) * nq + g.uyp * WPH, YP);
-        load_words<WPH>(sp32, (c ^ 2) * nq + g.uym * WPH, YM);
-        load_words<WPH>(sp32, (c ^ 2) * nq + g.uzp * WPH, ZP);
+        const int py = (c ^ 2) * nq;
+        const uint32_t S = sp32[c * nq + t];
+        const uint32_t YP = sp32[py + g.uyp * WPH + w], YM = sp32[py + g.uym * WPH + w];
+        const uint32_t ZP = sp32[py + g.uzp * WPH + w];
+        uint32_t Xp0, Xm0, Xp1, Xm1;
+        x_pair<WPH, HX>(sp32, (c ^ 1) * nq + u * WPH, w, Xp0, Xm0);
+        x_pair<WPH, HX>(sp32, (c ^ 3) * nq + g.uyp * WPH, w, Xp1, Xm1);
+        const int m4 = 2 * __builtin_popcount(S) - 4;
+        v[OBS_M] += m4;
+        v[OBS_SX] += HX ? -m4 : m4;
+        v[OBS_SY] += g.p ? -m4 : m4;
+        v[OBS_BX] += 4 - 2 * __builtin_popcount(S ^ Xp0);
+        v[OBS_BY] += 4 - 2 * __builtin_popcount(S ^ YP);
+        v[OBS_BZ] += 4 - 2 * __builtin_popcount(S ^ ZP);
+        v[OBS_BD] += 8 - 2 * (__builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1));
+        const uint32_t ax = (S ^ Xp0) + (S ^ Xm0), ay = (S ^ YP) + (S ^ YM);
+        const int64_t *qw = qb + 4 * (int64_t)(c * nq + t);
 #pragma unroll
-        for (int w = 0; w < WPH; ++w) {
-            const uint32_t Sw = S[w];
-            const int m4 = 2 * __builtin_popcount(Sw) - 4;
-            v[OBS_M] += m4;
-            v[OBS_SX] += HX ? -m4 : m4;
-            v[OBS_SY] += g.p ? -m4 : m4;
-            v[OBS_BX] += 4 - 2 * __builtin_popcount(Sw ^ Xp0[w]);
-            v[OBS_BY] += 4 - 2 * __builtin_popcount(Sw ^ YP[w]);
-            v[OBS_BZ] += 4 - 2 * __builtin_popcount(Sw ^ ZP[w]);
-            v[OBS_BD] += 8 - 2 * (__builtin_popcount(Sw ^ Xp1[w]) + __builtin_popcount(Sw ^ Xm1[w]));
-            const uint32_t ax = (Sw ^ Xp0[w]) + (Sw ^ Xm0[w]), ay = (Sw ^ YP[w]) + (Sw ^ YM[w]);
-            const int64_t *qw = qb + 4 * (int64_t)(c * nq + u * WPH + w);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int l = 2 * ((int)((ay >> (8 * j)) & 0xff) - (int)((ax >> (8 * j)) & 0xff));
-                v[OBS_EQ] += qw[j] * (int64_t)l;
-            }
+        for (int j = 0; j < 4; ++j) {
+            const int l = 2 * ((int)((ay >> (8 * j)) & 0xff) - (int)((ax >> (8 * j)) & 0xff));
+            v[OBS_EQ] += qw[j] * (int64_t)l;
         }
     }
 }
@@ -273,93 +234,63 @@ struct SweepCtx {
 };
 
 template <int WPH, int HX, bool SHADOW>
-__device__ __forceinline__ void update_unit(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c,
-                                            int u, int H, unsigned long long &sh_prop, unsigned long long &sh_mis)
+__device__ __forceinline__ void update_dword(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c,
+                                             int t, int H, unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
+    // t = u*WPH + w: dword w of unit (half-row) u of colour plane c, i.e. the
+    // sites with colour index q = 4t..4t+3 -- Philox group t.
     constexpr int HALF = 4 * WPH;
     const int nq = 4 * WPH * WPH * H;
+    const int u = t / WPH, w = t - u * WPH;
     const UnitGeo g = unit_geo<WPH>(u, c, H);
-    const int own = c * nq + u * WPH, px = (c ^ 1) * nq, py = (c ^ 2) * nq, pd = (c ^ 3) * nq;
-    // Only the reduced neighbour words stay live: Z = b(+z) + b(-z) and
-    // D = sum of the four diagonal b's, bytewise (<= 2 and <= 4: no carries).
-    uint32_t S[WPH], YP[WPH], YM[WPH], Z[WPH], D[WPH], Xp0[WPH], Xm0[WPH];
-    {
-        uint32_t T[WPH];
-        load_words<WPH>(lds32, py + g.uzp * WPH, Z);
-        load_words<WPH>(lds32, py + g.uzm * WPH, T);
+    const int own = c * nq + t, px = (c ^ 1) * nq + u * WPH, py = (c ^ 2) * nq, pd = (c ^ 3) * nq;
+    const uint32_t S = lds32[own];
+    const uint32_t YP = lds32[py + g.uyp * WPH + w], YM = lds32[py + g.uym * WPH + w];
+    const uint32_t ZP = lds32[py + g.uzp * WPH + w], ZM = lds32[py + g.uzm * WPH + w];
+    uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
+    x_pair<WPH, HX>(lds32, px, w, Xp0, Xm0);
+    x_pair<WPH, HX>(lds32, pd + g.uyp * WPH, w, Xp1, Xm1);
+    x_pair<WPH, HX>(lds32, pd + g.uym * WPH, w, Xp2, Xm2);
+    const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise, <= 4
+    const uint32_t Z = ZP + ZM;                         // bytewise, <= 2
+    const float4 kc = cx.kc4[own], kxp = cx.kxp4[own], kxm = cx.kxm4[own], kyp = cx.kyp4[own], kym = cx.kym4[own];
+    const U4 rn = philox4x32_10((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+    const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
+    const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
+    const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
+    const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
+    const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
+    const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+    uint32_t flip = 0;
 #pragma unroll
-        for (int w = 0; w < WPH; ++w) Z[w] += T[w];
-    }
-    {
-        uint32_t A[WPH], P[WPH], M[WPH];
-        load_words<WPH>(lds32, pd + g.uyp * WPH, A);
-        x_neighbours<WPH, HX>(A, P, M);
-#pragma unroll
-        for (int w = 0; w < WPH; ++w) D[w] = P[w] + M[w];
-        load_words<WPH>(lds32, pd + g.uym * WPH, A);
-        x_neighbours<WPH, HX>(A, P, M);
-#pragma unroll
-        for (int w = 0; w < WPH; ++w) D[w] += P[w] + M[w];
-        load_words<WPH>(lds32, px + u * WPH, A);
-        x_neighbours<WPH, HX>(A, Xp0, Xm0);
-    }
-    load_words<WPH>(lds32, py + g.uyp * WPH, YP);
-    load_words<WPH>(lds32, py + g.uym * WPH, YM);
-    load_words<WPH>(lds32, own, S);
-#pragma unroll
-    for (int w = 0; w < WPH; ++w) {
-        const int w4 = own + w;                           // float4 index of these 4 sites
-        const float4 kc = cx.kc4[w4], kxp = cx.kxp4[w4], kxm = cx.kxm4[w4], kyp = cx.kyp4[w4],
-                     kym = cx.kym4[w4];
-        const U4 rn = philox4x32_10((uint32_t)(u * WPH + w), cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
-        const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
-        const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
-        const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
-        const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
-        const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
-        const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
-        uint32_t flip = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float h = kca[j];
-            h = __builtin_fmaf(byte_f(Xp0[w], j), kxpa[j], h);
-            h = __builtin_fmaf(byte_f(Xm0[w], j), kxma[j], h);
-            h = __builtin_fmaf(byte_f(YP[w], j), kypa[j], h);
-            h = __builtin_fmaf(byte_f(YM[w], j), kyma[j], h);
-            h = __builtin_fmaf(byte_f(D[w], j), cx.j2x2, h);
-            h = __builtin_fmaf(byte_f(Z[w], j), cx.j3x2, h);
-            const bool up = byte_bit(S[w], j) != 0;
-            const bool acc = decide_f32(h, up, cx.b2, ra[j]);
-            if constexpr (SHADOW) {
-                // FP64 rule on the same state and draw; the individual
-                // neighbour words are re-read (no colour-c site changes them)
-                uint32_t A1[WPH], A2[WPH], P1[WPH], M1[WPH], P2[WPH], M2[WPH], ZP[WPH], ZM[WPH];
-                load_words<WPH>(lds32, pd + g.uyp * WPH, A1);
-                load_words<WPH>(lds32, pd + g.uym * WPH, A2);
-                x_neighbours<WPH, HX>(A1, P1, M1);
-                x_neighbours<WPH, HX>(A2, P2, M2);
-                load_words<WPH>(lds32, py + g.uzp * WPH, ZP);
-                load_words<WPH>(lds32, py + g.uzm * WPH, ZM);
-                const int m = 4 * w + j;                          // position in the half-row
-                const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
-                const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
-                const double sv[10] = {pm1(byte_bit(YP[w], j)), pm1(byte_bit(YM[w], j)),
-                                       pm1(byte_bit(Xp0[w], j)), pm1(byte_bit(Xm0[w], j)),
-                                       pm1(byte_bit(P1[w], j)), pm1(byte_bit(M1[w], j)),
-                                       pm1(byte_bit(P2[w], j)), pm1(byte_bit(M2[w], j)),
-                                       pm1(byte_bit(ZP[w], j)), pm1(byte_bit(ZM[w], j))};
-                const double pv[5] = {cx.p64[4 * own + m], cx.p64[4 * (py + g.uyp * WPH) + m],
-                                      cx.p64[4 * (py + g.uym * WPH) + m], cx.p64[4 * (px + u * WPH) + mo_p],
-                                      cx.p64[4 * (px + u * WPH) + mo_m]};
-                const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[j]);
-                sh_prop += 1;
-                sh_mis += (a64 != acc);
-            }
-            flip |= (acc ? 1u : 0u) << (8 * j);
+    for (int j = 0; j < 4; ++j) {
+        float h = kca[j];
+        h = __builtin_fmaf(byte_f(Xp0, j), kxpa[j], h);
+        h = __builtin_fmaf(byte_f(Xm0, j), kxma[j], h);
+        h = __builtin_fmaf(byte_f(YP, j), kypa[j], h);
+        h = __builtin_fmaf(byte_f(YM, j), kyma[j], h);
+        h = __builtin_fmaf(byte_f(D, j), cx.j2x2, h);
+        h = __builtin_fmaf(byte_f(Z, j), cx.j3x2, h);
+        const bool up = byte_bit(S, j) != 0;
+        const bool acc = decide_f32(h, up, cx.b2, ra[j]);
+        if constexpr (SHADOW) {
+            const int m = 4 * w + j;                          // position in the half-row
+            const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
+            const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
+            const double sv[10] = {pm1(byte_bit(YP, j)), pm1(byte_bit(YM, j)), pm1(byte_bit(Xp0, j)),
+                                   pm1(byte_bit(Xm0, j)), pm1(byte_bit(Xp1, j)), pm1(byte_bit(Xm1, j)),
+                                   pm1(byte_bit(Xp2, j)), pm1(byte_bit(Xm2, j)), pm1(byte_bit(ZP, j)),
+                                   pm1(byte_bit(ZM, j))};
+            const double pv[5] = {cx.p64[4 * own + j], cx.p64[4 * (py + g.uyp * WPH) + m],
+                                  cx.p64[4 * (py + g.uym * WPH) + m], cx.p64[4 * px + mo_p],
+                                  cx.p64[4 * px + mo_m]};
+            const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[j]);
+            sh_prop += 1;
+            sh_mis += (a64 != acc);
         }
-        S[w] ^= flip;
+        flip |= (acc ? 1u : 0u) << (8 * j);
     }
-    store_words<WPH>(lds32, own, S);
+    lds32[own] = S ^ flip;
 }
 
 template <int WPH, int NT, bool SHADOW>
@@ -410,9 +341,10 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
-            for (int u = tid; u < nunits; u += NT) {
-                if (c & 1) update_unit<WPH, 1, SHADOW>(lds32, cx, cc, c, u, H, sh_prop, sh_mis);
-                else update_unit<WPH, 0, SHADOW>(lds32, cx, cc, c, u, H, sh_prop, sh_mis);
+#pragma unroll 1
+            for (int t = tid; t < nunits * WPH; t += NT) {
+                if (c & 1) update_dword<WPH, 1, SHADOW>(lds32, cx, cc, c, t, H, sh_prop, sh_mis);
+                else update_dword<WPH, 0, SHADOW>(lds32, cx, cc, c, t, H, sh_prop, sh_mis);
             }
             __syncthreads();
         }
