@@ -87,8 +87,9 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
     for (int r = 0; r < 10; ++r) {
         uint64_t p0 = (uint64_t)0xD2511F53u * c0;
         uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
-        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
-        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        // three-input XOR in one v_bitop3 (truth table 0x96 is symmetric in its inputs)
+        uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
         c1 = (uint32_t)p1;
         c3 = (uint32_t)p0;
         c0 = n0;
@@ -100,14 +101,12 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 }
 
 // ---------------------------------------------------------------------------
-// Acceptance threshold of the FP32 rule: 2^24 * 2^y for y >= -64.
+// Acceptance threshold of the FP32 rule: 2^24 * 2^y for y in [-64, 0].
 // y = n + f, n = rint(y), |f| <= 1/2, 2^f by a degree-6 near-minimax
 // polynomial (relative error < 1.6e-8) with coefficients pre-scaled by 2^24.
-// A proposal with uphill energy is accepted iff float(r >> 8) < threshold.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float accept_threshold_f32(float y)
+__device__ __forceinline__ float threshold_clamped(float y)
 {
-    y = fmaxf(y, -64.0f);
     float n = rintf(y);
     float f = y - n;
     float p = 0x1.41d332p+11f;
@@ -118,6 +117,20 @@ __device__ __forceinline__ float accept_threshold_f32(float y)
     p = __builtin_fmaf(p, f, 0x1.62e430p+23f);
     p = __builtin_fmaf(p, f, 0x1.000000p+24f);
     return ldexpf(p, (int)n);
+}
+
+// Metropolis decision of the FP32 rule for the site held in byte j of the
+// spin word `sw` (b = 1 for s = +1), local field h, b2 = 2 beta log2(e), and
+// Philox word r.  y = -b2 s h; a move with y >= 0 (dE <= 0) is always taken,
+// otherwise it is taken iff u24 = r >> 8 < 2^24 2^max(y, -64).  Clamping y to
+// [-64, 0] folds both cases into one comparison: y = 0 gives the threshold
+// 2^24, above every u24.
+__device__ __forceinline__ bool decide_f32_word(float h, uint32_t sw, int j, float b2, uint32_t r)
+{
+    const float g = b2 * h;
+    const uint32_t sgn = (sw << (31 - 8 * j)) & 0x80000000u;
+    const float y = __builtin_amdgcn_fmed3f(__uint_as_float(__float_as_uint(g) ^ sgn), -64.0f, 0.0f);
+    return (float)(r >> 8) < threshold_clamped(y);
 }
 
 // ---------------------------------------------------------------------------

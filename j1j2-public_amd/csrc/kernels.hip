@@ -25,13 +25,6 @@ constexpr int GLB_THREADS = 256;
 __device__ __forceinline__ uint32_t byte_bit(uint32_t w, int j) { return (w >> (8 * j)) & 1u; }
 __device__ __forceinline__ float byte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xffu); }
 
-// FP32 decision: uphill iff s*h > 0; accept iff not uphill or u24 < 2^24 2^(-b2 s h)
-__device__ __forceinline__ bool decide_f32(float h, bool up, float b2, uint32_t r)
-{
-    float sh = up ? h : -h;
-    float thr = accept_threshold_f32(-(b2 * sh));
-    return !(sh > 0.0f) || ((float)(r >> 8) < thr);
-}
 
 // FP64 decision of Metro_sweep (J1J2_6tv.f90:2572-2604) with the engine's uniform
 __device__ __forceinline__ bool decide_f64(double field, bool up, double beta, uint32_t r)
@@ -233,24 +226,59 @@ struct SweepCtx {
     uint32_t ctr1, sweep_lo, sweep_hi, key0, key1;
 };
 
-template <int WPH, int HX, bool SHADOW>
-__device__ __forceinline__ void update_dword(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c,
-                                             int t, int H, unsigned long long &sh_prop, unsigned long long &sh_mis)
+// Offsets (in dwords, relative to the updated word) of the neighbour words of
+// one colour-plane dword: +-y, +-z (plane c^2), the other half of the row
+// (plane c^1, words w and w-+1) and of rows y+-1 (plane c^3).
+struct NbOff {
+    int yp, ym, zp, zm, xa, xb, d1a, d1b, d2a, d2b;
+};
+
+template <int WPH, int HX>
+__device__ __forceinline__ NbOff nb_offsets(int c, int nq, int u, int w, const UnitGeo &g)
 {
-    // t = u*WPH + w: dword w of unit (half-row) u of colour plane c, i.e. the
-    // sites with colour index q = 4t..4t+3 -- Philox group t.
+    const int wb = HX ? (w == WPH - 1 ? 0 : w + 1) : (w == 0 ? WPH - 1 : w - 1);
+    const int dY = ((c ^ 2) - c) * nq, dX = ((c ^ 1) - c) * nq, dD = ((c ^ 3) - c) * nq;
+    NbOff o;
+    o.yp = dY + (g.uyp - u) * WPH;
+    o.ym = dY + (g.uym - u) * WPH;
+    o.zp = dY + (g.uzp - u) * WPH;
+    o.zm = dY + (g.uzm - u) * WPH;
+    o.xa = dX;
+    o.xb = dX + (wb - w);
+    o.d1a = dD + (g.uyp - u) * WPH;
+    o.d1b = o.d1a + (wb - w);
+    o.d2a = dD + (g.uym - u) * WPH;
+    o.d2b = o.d2a + (wb - w);
+    return o;
+}
+
+template <int HX>
+__device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uint32_t &Xm)
+{
+    if constexpr (HX == 0) {      // even sites: +x = odd m, -x = odd m-1
+        Xp = A;
+        Xm = __builtin_amdgcn_alignbyte(A, B, 3);
+    } else {                      // odd sites: -x = even m, +x = even m+1
+        Xm = A;
+        Xp = __builtin_amdgcn_alignbyte(B, A, 1);
+    }
+}
+
+// Update the 4 sites of colour-plane dword `own` (plane-relative index t =
+// u*WPH + w, i.e. colour indices 4t..4t+3, Philox group t).
+template <int WPH, int HX, bool SHADOW>
+__device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int own,
+                                           int t, int w, const NbOff &o, unsigned long long &sh_prop,
+                                           unsigned long long &sh_mis)
+{
     constexpr int HALF = 4 * WPH;
-    const int nq = 4 * WPH * WPH * H;
-    const int u = t / WPH, w = t - u * WPH;
-    const UnitGeo g = unit_geo<WPH>(u, c, H);
-    const int own = c * nq + t, px = (c ^ 1) * nq + u * WPH, py = (c ^ 2) * nq, pd = (c ^ 3) * nq;
     const uint32_t S = lds32[own];
-    const uint32_t YP = lds32[py + g.uyp * WPH + w], YM = lds32[py + g.uym * WPH + w];
-    const uint32_t ZP = lds32[py + g.uzp * WPH + w], ZM = lds32[py + g.uzm * WPH + w];
+    const uint32_t YP = lds32[own + o.yp], YM = lds32[own + o.ym];
+    const uint32_t ZP = lds32[own + o.zp], ZM = lds32[own + o.zm];
     uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
-    x_pair<WPH, HX>(lds32, px, w, Xp0, Xm0);
-    x_pair<WPH, HX>(lds32, pd + g.uyp * WPH, w, Xp1, Xm1);
-    x_pair<WPH, HX>(lds32, pd + g.uym * WPH, w, Xp2, Xm2);
+    x_from<HX>(lds32[own + o.xa], lds32[own + o.xb], Xp0, Xm0);
+    x_from<HX>(lds32[own + o.d1a], lds32[own + o.d1b], Xp1, Xm1);
+    x_from<HX>(lds32[own + o.d2a], lds32[own + o.d2b], Xp2, Xm2);
     const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise, <= 4
     const uint32_t Z = ZP + ZM;                         // bytewise, <= 2
     const float4 kc = cx.kc4[own], kxp = cx.kxp4[own], kxm = cx.kxm4[own], kyp = cx.kyp4[own], kym = cx.kym4[own];
@@ -271,9 +299,9 @@ __device__ __forceinline__ void update_dword(uint32_t *lds32, const SweepCtx &cx
         h = __builtin_fmaf(byte_f(YM, j), kyma[j], h);
         h = __builtin_fmaf(byte_f(D, j), cx.j2x2, h);
         h = __builtin_fmaf(byte_f(Z, j), cx.j3x2, h);
-        const bool up = byte_bit(S, j) != 0;
-        const bool acc = decide_f32(h, up, cx.b2, ra[j]);
+        const bool acc = decide_f32_word(h, S, j, cx.b2, ra[j]);
         if constexpr (SHADOW) {
+            const bool up = byte_bit(S, j) != 0;
             const int m = 4 * w + j;                          // position in the half-row
             const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
             const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
@@ -281,9 +309,9 @@ __device__ __forceinline__ void update_dword(uint32_t *lds32, const SweepCtx &cx
                                    pm1(byte_bit(Xm0, j)), pm1(byte_bit(Xp1, j)), pm1(byte_bit(Xm1, j)),
                                    pm1(byte_bit(Xp2, j)), pm1(byte_bit(Xm2, j)), pm1(byte_bit(ZP, j)),
                                    pm1(byte_bit(ZM, j))};
-            const double pv[5] = {cx.p64[4 * own + j], cx.p64[4 * (py + g.uyp * WPH) + m],
-                                  cx.p64[4 * (py + g.uym * WPH) + m], cx.p64[4 * px + mo_p],
-                                  cx.p64[4 * px + mo_m]};
+            const int xbase = 4 * (own + o.xa - w);           // first site of the other half-row
+            const double pv[5] = {cx.p64[4 * own + j], cx.p64[4 * (own + o.yp) + j], cx.p64[4 * (own + o.ym) + j],
+                                  cx.p64[xbase + mo_p], cx.p64[xbase + mo_m]};
             const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[j]);
             sh_prop += 1;
             sh_mis += (a64 != acc);
@@ -293,11 +321,43 @@ __device__ __forceinline__ void update_dword(uint32_t *lds32, const SweepCtx &cx
     lds32[own] = S ^ flip;
 }
 
+// One colour pass over plane c.
+template <int WPH, int NT, int HX, bool SHADOW>
+__device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
+                                            unsigned long long &sh_prop, unsigned long long &sh_mis)
+{
+    constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH;    // dwords per z-layer of a plane
+    const int nq = PER_Z * H, tid = threadIdx.x;
+    if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) {
+        // Each thread keeps its (y/2, w) and steps z by NT/PER_Z (even, so y's
+        // parity is fixed): every neighbour offset is constant except at the
+        // z boundary.
+        constexpr int ZSTEP = NT / PER_Z;
+        const int tz = tid % PER_Z, w = tz % WPH, z0 = tid / PER_Z;
+        const int u0 = z0 * HALF + tz / WPH;
+        const UnitGeo g0 = unit_geo<WPH>(u0, c, H);
+        NbOff o = nb_offsets<WPH, HX>(c, nq, u0, w, g0);
+        const int dY = ((c ^ 2) - c) * nq;
+        for (int z = z0; z < H; z += ZSTEP) {
+            const int t = z * PER_Z + tz;
+            o.zp = dY + (z == H - 1 ? -(H - 1) * PER_Z : PER_Z);
+            o.zm = dY + (z == 0 ? (H - 1) * PER_Z : -PER_Z);
+            update_word<WPH, HX, SHADOW>(lds32, cx, cc, c * nq + t, t, w, o, sh_prop, sh_mis);
+        }
+    } else {
+        for (int t = tid; t < nq; t += NT) {
+            const int u = t / WPH, w = t - u * WPH;
+            const UnitGeo g = unit_geo<WPH>(u, c, H);
+            const NbOff o = nb_offsets<WPH, HX>(c, nq, u, w, g);
+            update_word<WPH, HX, SHADOW>(lds32, cx, cc, c * nq + t, t, w, o, sh_prop, sh_mis);
+        }
+    }
+}
+
 template <int WPH, int NT, bool SHADOW>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
                                               uint32_t *lds32, int64_t (*red)[OBS_N])
 {
-    constexpr int HALF = 4 * WPH;
     const int H = d.H;
     const int tid = threadIdx.x;
     const int N = 64 * WPH * WPH * H;
@@ -323,7 +383,6 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
     const uint64_t S0 = st.clk->cycle * (uint64_t)st.ptswapever;
-    const int nunits = HALF * H;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
     // the previous cycle measured and left this replica at slot PTMEASINDEX:
@@ -341,11 +400,8 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
-#pragma unroll 1
-            for (int t = tid; t < nunits * WPH; t += NT) {
-                if (c & 1) update_dword<WPH, 1, SHADOW>(lds32, cx, cc, c, t, H, sh_prop, sh_mis);
-                else update_dword<WPH, 0, SHADOW>(lds32, cx, cc, c, t, H, sh_prop, sh_mis);
-            }
+            if (c & 1) colour_pass<WPH, NT, 1, SHADOW>(lds32, cx, cc, c, H, sh_prop, sh_mis);
+            else colour_pass<WPH, NT, 0, SHADOW>(lds32, cx, cc, c, H, sh_prop, sh_mis);
             __syncthreads();
         }
     }
@@ -495,7 +551,7 @@ __global__ __launch_bounds__(GLB_THREADS) void k_sweep_global(DeviceState st, in
             h = __builtin_fmaf((float)b[1], kym[nb.i], h);
             h = __builtin_fmaf((float)(b[4] + b[5] + b[6] + b[7]), cc.j2x2, h);
             h = __builtin_fmaf((float)(b[8] + b[9]), cc.j3x2, h);
-            const bool a32 = decide_f32(h, up != 0, b2, ra[j]);
+            const bool a32 = decide_f32_word(h, up, 0, b2, ra[j]);
             if (PREC == 1) acc = a32; else other = a32;
         }
         if (PREC == 0 || SHADOW) {
