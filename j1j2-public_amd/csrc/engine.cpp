@@ -69,8 +69,6 @@ struct Engine {
     bool observed = false;                     // observables valid for the current state
     std::vector<void *> allocs;
     DeviceState st{};
-    DeviceState st_launch{};                   // st with the per-launch PT-step mode
-    size_t res_lds = 0;                        // dynamic LDS of the resident launch
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     Timer timers[4];
 
@@ -311,12 +309,8 @@ int enqueue_cycle(Engine &E, int mode)
     int rc;
     if (!values_only) {
         if (E.all_resident) {
-            // one launch: every lattice's sweeps, its observables, and (by the
-            // last replica block of each config) the config's PT step
-            E.st_launch = E.st;
-            E.st_launch.pt_mode = measure ? 1 : 0;
             auto fn = [](Engine &e, const void *) {
-                return launch_sweep_resident(e.st_launch, 0, e.st.nlat, e.res_lds, e.uniform_L, e.res_nt, e.shadow,
+                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.uniform_L, e.res_nt, e.shadow,
                                              e.stream);
             };
             if ((rc = timed(E, 0, fn, nullptr))) return rc;
@@ -359,7 +353,7 @@ int enqueue_cycle(Engine &E, int mode)
     }
     auto ptfn = [](Engine &e, const void *a) { return launch_pt_step(e.st, *static_cast<const int *>(a), e.stream); };
     const int ptmode = values_only ? 2 : (measure ? 1 : 0);
-    if ((values_only || !E.st.fuse_pt) && (rc = timed(E, 1, ptfn, &ptmode))) return rc;
+    if ((rc = timed(E, 1, ptfn, &ptmode))) return rc;
     if (values_only) return 0;
     if (measure && E.locnem && !E.st.fuse_locnem) {
         auto fn = [](Engine &e, const void *) { return launch_locnem(e.st, e.max_n, 0, e.stream); };
@@ -544,7 +538,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         if (G.p.L != c0.L) E->uniform_L = 0;
     if (const char *nt = std::getenv("J1J2_RES_THREADS")) E->res_nt = std::atoi(nt) == 512 ? 512 : 256;
     if (E->all_resident || std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.resident; }))
-        HIPCHK(setup_resident(std::max((size_t)E->max_n, pt_step_lds(256))));
+        HIPCHK(setup_resident((size_t)E->max_n));
 
     DeviceState &st = E->st;
     const int64_t nconf = (int64_t)E->confs.size();
@@ -566,7 +560,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         (rc = E->alloc(st.locspin, E->nsites)) || (rc = E->alloc(st.locnem, E->nsites)) ||
         (rc = E->alloc(st.locsamples, nconf)) || (rc = E->alloc(st.series, E->nseries)) ||
         (rc = E->alloc(st.series_len, nconf)) || (rc = E->alloc(st.shadow, 2 * nconf)) ||
-        (rc = E->alloc(st.locnem_pending, nconf)) || (rc = E->alloc(st.conf_done, nconf)))
+        (rc = E->alloc(st.locnem_pending, nconf)))
         return rc;
     st.flips = nullptr;
     if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
@@ -578,10 +572,6 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;
     st.seed = (uint32_t)c0.seed;
     st.fuse_locnem = (E->locnem && E->all_resident) ? 1 : 0;
-    st.fuse_pt = E->all_resident ? 1 : 0;
-    int32_t max_nrep = 1;
-    for (const ConfDesc &cd : E->confs) max_nrep = std::max(max_nrep, cd.nrep);
-    E->res_lds = std::max((size_t)E->max_n, pt_step_lds(max_nrep));
     if ((rc = h2d(*E, lats, E->lats)) || (rc = h2d(*E, confs, E->confs)) || (rc = h2d(*E, cconst, E->cconst)))
         return rc;
     std::vector<double> zero_p((size_t)E->max_n, 0.0);

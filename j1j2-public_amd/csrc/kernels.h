@@ -48,9 +48,6 @@ struct DeviceState {
     int64_t *locsamples;       // nconf
     int32_t *locnem_pending;   // nconf: the last cycle measured; its state is still to be accumulated
     int32_t fuse_locnem;       // 1: the resident sweep accumulates pending local-nematic sums
-    int32_t fuse_pt;           // 1: the resident sweep ends in the PT step of each config
-    int32_t pt_mode;           // PT-step mode of a fused launch (bit 0: measure)
-    uint32_t *conf_done;       // nconf: replica blocks of the config finished this cycle
     double *series;            // LOCAL series
     int64_t *series_len;       // nconf
     int64_t series_cap;
@@ -69,8 +66,6 @@ hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32
                                  int32_t max_n, int32_t L, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);
 size_t resident_lds_limit();
-// LDS the fused PT step needs for a config of nrep replicas.
-size_t pt_step_lds(int32_t nrep);
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).
 hipError_t setup_resident(size_t max_lds);
 // Global-memory FP32 sweep, one colour pass of sweep k of the current cycle

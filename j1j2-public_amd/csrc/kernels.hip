@@ -212,158 +212,6 @@ __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H
 }
 
 // ---------------------------------------------------------------------------
-// Per-config PT step (one wave per config): energies from the integer
-// observables, one pass of adjacent swaps (Pt_attempt_swaps, J1J2_6tv.f90:
-// 1303-1338; pairs are disjoint so they are decided in parallel), then the
-// per-slot accumulation of Pt_measurement_loop (J1J2_6tv.f90:1480-1524).
-// ---------------------------------------------------------------------------
-constexpr int PT_THREADS = 64;
-constexpr int MAX_NREP = 256;
-
-// Shared-memory working set of the PT step for one config.
-struct PtShared {
-    int32_t *rat, *tor;
-    double *et;
-    double (*vv)[5];
-};
-
-__device__ __forceinline__ size_t pt_shared_bytes(int nrep) { return (size_t)nrep * (4 + 4 + 8 + 40); }
-
-__device__ __forceinline__ PtShared pt_shared_from(void *base, int nrep)
-{
-    PtShared sh;
-    char *p = static_cast<char *>(base);
-    sh.et = reinterpret_cast<double *>(p);
-    sh.vv = reinterpret_cast<double (*)[5]>(p + 8 * nrep);
-    sh.rat = reinterpret_cast<int32_t *>(p + 48 * nrep);
-    sh.tor = reinterpret_cast<int32_t *>(p + 52 * nrep);
-    return sh;
-}
-
-// The PT step of one config, executed by all nthr threads of one block.
-// mode bit 0: accumulate measurements; bit 1: energies and values only (no
-// swaps, no clock advance).  The last config to finish a step advances the
-// cycle clock (every block of the cycle has read it before that).
-__device__ void pt_step_body(const DeviceState &st, int conf, int mode, int tid, int nthr, const PtShared &sh)
-{
-    const bool measure = (mode & 1) != 0, swaps = (mode & 2) == 0;
-    int32_t *rat = sh.rat, *tor = sh.tor;
-    double *et = sh.et;
-    double(*vv)[5] = sh.vv;
-    const ConfDesc cd = st.confs[conf];
-    const ConfConst cc = st.cconst[conf];
-    const int nrep = cd.nrep, sb = cd.slot_base;
-    const double N = (double)cd.N;
-    const uint64_t cycle = st.clk->cycle;
-    const int phase = ((cycle - st.clk->phase_start) & 1ull) ? 2 : 1;
-    const int64_t slen = st.local ? st.series_len[conf] : 0;
-    for (int r = tid; r < nrep; r += nthr) {
-        const int64_t *o = st.obs + (int64_t)(cd.lat0 + r) * OBS_N;
-        const double ej = ((cc.J1h * (double)o[OBS_BX]) + (cc.J1v * (double)o[OBS_BY])) +
-                          ((cc.J2 * (double)o[OBS_BD]) + (cc.J3 * (double)o[OBS_BZ]));
-        const double E = -ej - (double)o[OBS_EQ] * 0x1p-32;
-        et[r] = E;
-        vv[r][0] = (double)o[OBS_M] / N;
-        vv[r][1] = E / N;
-        vv[r][2] = (double)o[OBS_SX] / N;
-        vv[r][3] = (double)o[OBS_SY] / N;
-        vv[r][4] = (double)(2 * (o[OBS_BX] - o[OBS_BY])) / N;
-        st.etot[sb + r] = E;
-        for (int q = 0; q < 5; ++q) st.vals[(int64_t)(sb + r) * 5 + q] = vv[r][q];
-        rat[r] = st.rep_at_temp[sb + r];
-        tor[r] = st.temp_of_rep[sb + r];
-    }
-    __syncthreads();
-    if (!st.local && swaps) {
-        const int start = (phase == 1) ? 0 : 1;
-        for (int s = start + 2 * tid; s < nrep - 1; s += 2 * nthr) {
-            const int rlo = rat[s], rhi = rat[s + 1];
-            st.att[cd.pair_base + s] += 1;
-            const double delta = (st.beta[sb + s] - st.beta[sb + s + 1]) * (et[rlo] - et[rhi]);
-            const U4 rn = philox4x32_10((uint32_t)s, 0x80000000u, (uint32_t)cycle, (uint32_t)(cycle >> 32),
-                                        st.seed, (uint32_t)cd.iconf);
-            const double u = (double)rn.x * 0x1p-32;
-            if (delta >= 0.0 || u < exp64(delta)) {
-                st.acc[cd.pair_base + s] += 1;
-                rat[s] = rhi;
-                rat[s + 1] = rlo;
-                tor[rlo] = s + 1;
-                tor[rhi] = s;
-            }
-        }
-        __syncthreads();
-        for (int r = tid; r < nrep; r += nthr) {
-            st.rep_at_temp[sb + r] = rat[r];
-            st.temp_of_rep[sb + r] = tor[r];
-        }
-    }
-    if (measure) {
-        for (int sl = tid; sl < nrep; sl += nthr) {
-            if (!st.local) {
-                if (tor[sl] == 0) st.vcold[sb + sl] = 1;           // sl as a replica index
-                if (tor[sl] == nrep - 1) st.vhot[sb + sl] = 1;
-            }
-            const int r = rat[sl];
-            const double am = vv[r][0], ae = vv[r][1], ax = vv[r][2], ay = vv[r][3], an = vv[r][4];
-            if (st.local) {
-                if (slen < st.series_cap) {
-                    double *sr = st.series + cd.series_off + (int64_t)sl * 5 * st.series_cap;
-                    for (int q = 0; q < 5; ++q) sr[(int64_t)q * st.series_cap + slen] = vv[r][q];
-                }
-                continue;
-            }
-            const double ax2 = ax * ax, ay2 = ay * ay, m2 = am * am, e2 = ae * ae, n2 = an * an;
-            const double str2 = ax2 + ay2;
-            double *S = st.sums + (int64_t)(sb + sl) * NSUMS;
-            S[0] += am;      S[1] += fabs(am);      S[2] += m2;         S[3] += m2 * m2;
-            S[4] += ae;      S[5] += e2;            S[6] += fabs(ax);   S[7] += fabs(ay);
-            S[8] += ax2;     S[9] += ax2 * ax2;     S[10] += ay2;       S[11] += ay2 * ay2;
-            S[12] += str2;   S[13] += str2 * str2;  S[14] += sqrt(str2); S[15] += fabs(ay2 - ax2);
-            S[16] += an;     S[17] += fabs(an);     S[18] += n2;        S[19] += n2 * n2;
-            if (ae < st.best[sb + sl]) st.best[sb + sl] = ae;
-            st.final_en[sb + sl] = ae;
-            if (st.hbin[sb + sl] <= 0.0) {
-                st.hmin[sb + sl] = ae - (fabs(ae) + 1.0);
-                st.hmax[sb + sl] = ae + (fabs(ae) + 1.0);
-                st.hbin[sb + sl] = (st.hmax[sb + sl] - st.hmin[sb + sl]) / (double)st.bins;
-            }
-            const double tq = (ae - st.hmin[sb + sl]) / st.hbin[sb + sl];
-            int idx;
-            if (tq < 0.0) idx = 1;
-            else if (tq >= (double)st.bins) idx = st.bins;
-            else idx = 1 + (int)tq;
-            idx = idx < 1 ? 1 : (idx > st.bins ? st.bins : idx);
-            st.hist[cd.hist_off + (int64_t)sl * st.bins + idx - 1] += 1;
-        }
-        if (tid == 0) {
-            st.nmeasure[conf] += 1;
-            if (st.local) st.series_len[conf] = slen + 1;
-        }
-    }
-    if (swaps) {
-        if (tid == 0 && st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
-        __syncthreads();
-        if (tid == 0) {
-            __threadfence();
-            const unsigned prev = atomicAdd(&st.clk->done, 1u);
-            if (prev == (unsigned)st.nconf - 1) {
-                st.clk->done = 0;
-                st.clk->cycle = cycle + 1;
-                __threadfence();
-            }
-        }
-    }
-}
-
-// Stand-alone PT step (paths whose sweep does not end in it), one block per config.
-__global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t mode)
-{
-    __shared__ double smem[MAX_NREP * 7];
-    const PtShared sh = pt_shared_from(smem, st.confs[blockIdx.x].nrep);
-    pt_step_body(st, blockIdx.x, mode, threadIdx.x, PT_THREADS, sh);
-}
-
-// ---------------------------------------------------------------------------
 // LDS-resident FP32 sweep.  One block per lattice; the lattice's spins live in
 // LDS for all PTSWAPEVER sweeps of the cycle, and the per-site FP32 constants
 // stream from L2/HBM as float4 (one per plane per 4 sites).  In a colour pass
@@ -404,13 +252,6 @@ __device__ __forceinline__ NbOff nb_offsets(int c, int nq, int u, int w, const U
     return o;
 }
 
-// float4 load at a 32-bit byte offset from a block-uniform base, so the load
-// can address as SGPR base + VGPR offset (no 64-bit address arithmetic).
-__device__ __forceinline__ float4 ld16(const float4 *base, uint32_t byte_off)
-{
-    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
-}
-
 template <int HX>
 __device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uint32_t &Xm)
 {
@@ -440,9 +281,7 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     x_from<HX>(lds32[own + o.d2a], lds32[own + o.d2b], Xp2, Xm2);
     const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise, <= 4
     const uint32_t Z = ZP + ZM;                         // bytewise, <= 2
-    const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
-    const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob), kyp = ld16(cx.kyp4, ob),
-                 kym = ld16(cx.kym4, ob);
+    const float4 kc = cx.kc4[own], kxp = cx.kxp4[own], kxm = cx.kxm4[own], kyp = cx.kyp4[own], kym = cx.kym4[own];
     const U4 rn = philox4x32_10((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
     const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
     const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
@@ -589,21 +428,6 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
         for (int k = 0; k < OBS_N; ++k) o[k] = v[k];
-    }
-    if (st.fuse_pt) {
-        // The last replica of the config to finish runs the config's PT step
-        // (threadfence reduction: publish obs, count, then acquire).
-        __shared__ int last;
-        if (tid == 0) {
-            __threadfence();
-            last = atomicAdd(&st.conf_done[d.conf], 1u) == (unsigned)cd.nrep - 1;
-        }
-        __syncthreads();
-        if (last) {
-            __threadfence();
-            if (tid == 0) st.conf_done[d.conf] = 0;
-            pt_step_body(st, d.conf, st.pt_mode, tid, NT, pt_shared_from(lds32, cd.nrep));
-        }
     }
 }
 
@@ -793,6 +617,129 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
 }
 
 // ---------------------------------------------------------------------------
+// Per-config PT step (one wave per config): energies from the integer
+// observables, one pass of adjacent swaps (Pt_attempt_swaps, J1J2_6tv.f90:
+// 1303-1338; pairs are disjoint so they are decided in parallel), then the
+// per-slot accumulation of Pt_measurement_loop (J1J2_6tv.f90:1480-1524).
+// ---------------------------------------------------------------------------
+constexpr int PT_THREADS = 64;
+constexpr int MAX_NREP = 256;
+
+__global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t mode)
+{
+    // mode bit 0: accumulate measurements; bit 1: energies and values only
+    const bool measure = (mode & 1) != 0, swaps = (mode & 2) == 0;
+    __shared__ int32_t rat[MAX_NREP], tor[MAX_NREP];
+    __shared__ double et[MAX_NREP], vv[MAX_NREP][5];
+    const int conf = blockIdx.x, tid = threadIdx.x;
+    const ConfDesc cd = st.confs[conf];
+    const ConfConst cc = st.cconst[conf];
+    const int nrep = cd.nrep, sb = cd.slot_base;
+    const double N = (double)cd.N;
+    const uint64_t cycle = st.clk->cycle;
+    const int phase = ((cycle - st.clk->phase_start) & 1ull) ? 2 : 1;
+    const int64_t slen = st.local ? st.series_len[conf] : 0;
+    for (int r = tid; r < nrep; r += PT_THREADS) {
+        const int64_t *o = st.obs + (int64_t)(cd.lat0 + r) * OBS_N;
+        const double ej = ((cc.J1h * (double)o[OBS_BX]) + (cc.J1v * (double)o[OBS_BY])) +
+                          ((cc.J2 * (double)o[OBS_BD]) + (cc.J3 * (double)o[OBS_BZ]));
+        const double E = -ej - (double)o[OBS_EQ] * 0x1p-32;
+        et[r] = E;
+        vv[r][0] = (double)o[OBS_M] / N;
+        vv[r][1] = E / N;
+        vv[r][2] = (double)o[OBS_SX] / N;
+        vv[r][3] = (double)o[OBS_SY] / N;
+        vv[r][4] = (double)(2 * (o[OBS_BX] - o[OBS_BY])) / N;
+        st.etot[sb + r] = E;
+        for (int q = 0; q < 5; ++q) st.vals[(int64_t)(sb + r) * 5 + q] = vv[r][q];
+        rat[r] = st.rep_at_temp[sb + r];
+        tor[r] = st.temp_of_rep[sb + r];
+    }
+    __syncthreads();
+    if (!st.local && swaps) {
+        const int start = (phase == 1) ? 0 : 1;
+        for (int s = start + 2 * tid; s < nrep - 1; s += 2 * PT_THREADS) {
+            const int rlo = rat[s], rhi = rat[s + 1];
+            st.att[cd.pair_base + s] += 1;
+            const double delta = (st.beta[sb + s] - st.beta[sb + s + 1]) * (et[rlo] - et[rhi]);
+            const U4 rn = philox4x32_10((uint32_t)s, 0x80000000u, (uint32_t)cycle, (uint32_t)(cycle >> 32),
+                                        st.seed, (uint32_t)cd.iconf);
+            const double u = (double)rn.x * 0x1p-32;
+            if (delta >= 0.0 || u < exp64(delta)) {
+                st.acc[cd.pair_base + s] += 1;
+                rat[s] = rhi;
+                rat[s + 1] = rlo;
+                tor[rlo] = s + 1;
+                tor[rhi] = s;
+            }
+        }
+        __syncthreads();
+        for (int r = tid; r < nrep; r += PT_THREADS) {
+            st.rep_at_temp[sb + r] = rat[r];
+            st.temp_of_rep[sb + r] = tor[r];
+        }
+    }
+    if (measure) {
+        for (int sl = tid; sl < nrep; sl += PT_THREADS) {
+            if (!st.local) {
+                if (tor[sl] == 0) st.vcold[sb + sl] = 1;           // sl as a replica index
+                if (tor[sl] == nrep - 1) st.vhot[sb + sl] = 1;
+            }
+            const int r = rat[sl];
+            const double am = vv[r][0], ae = vv[r][1], ax = vv[r][2], ay = vv[r][3], an = vv[r][4];
+            if (st.local) {
+                if (slen < st.series_cap) {
+                    double *sr = st.series + cd.series_off + (int64_t)sl * 5 * st.series_cap;
+                    for (int q = 0; q < 5; ++q) sr[(int64_t)q * st.series_cap + slen] = vv[r][q];
+                }
+                continue;
+            }
+            const double ax2 = ax * ax, ay2 = ay * ay, m2 = am * am, e2 = ae * ae, n2 = an * an;
+            const double str2 = ax2 + ay2;
+            double *S = st.sums + (int64_t)(sb + sl) * NSUMS;
+            S[0] += am;      S[1] += fabs(am);      S[2] += m2;         S[3] += m2 * m2;
+            S[4] += ae;      S[5] += e2;            S[6] += fabs(ax);   S[7] += fabs(ay);
+            S[8] += ax2;     S[9] += ax2 * ax2;     S[10] += ay2;       S[11] += ay2 * ay2;
+            S[12] += str2;   S[13] += str2 * str2;  S[14] += sqrt(str2); S[15] += fabs(ay2 - ax2);
+            S[16] += an;     S[17] += fabs(an);     S[18] += n2;        S[19] += n2 * n2;
+            if (ae < st.best[sb + sl]) st.best[sb + sl] = ae;
+            st.final_en[sb + sl] = ae;
+            if (st.hbin[sb + sl] <= 0.0) {
+                st.hmin[sb + sl] = ae - (fabs(ae) + 1.0);
+                st.hmax[sb + sl] = ae + (fabs(ae) + 1.0);
+                st.hbin[sb + sl] = (st.hmax[sb + sl] - st.hmin[sb + sl]) / (double)st.bins;
+            }
+            const double tq = (ae - st.hmin[sb + sl]) / st.hbin[sb + sl];
+            int idx;
+            if (tq < 0.0) idx = 1;
+            else if (tq >= (double)st.bins) idx = st.bins;
+            else idx = 1 + (int)tq;
+            idx = idx < 1 ? 1 : (idx > st.bins ? st.bins : idx);
+            st.hist[cd.hist_off + (int64_t)sl * st.bins + idx - 1] += 1;
+        }
+        if (tid == 0) {
+            st.nmeasure[conf] += 1;
+            if (st.local) st.series_len[conf] = slen + 1;
+        }
+    }
+    if (swaps) {
+        if (tid == 0 && st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
+        // the last block to finish advances the cycle clock (every block has
+        // read it before incrementing `done`)
+        __syncthreads();
+        if (tid == 0) {
+            __threadfence();
+            const unsigned prev = atomicAdd(&st.clk->done, 1u);
+            if (prev == gridDim.x - 1) {
+                st.clk->done = 0;
+                st.clk->cycle = cycle + 1;
+                __threadfence();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Local nematic accumulation (J1J2_6tv.f90:1009-1025) for the replica at slot
 // PTMEASINDEX, as exact integer sums, from global memory.
 // ---------------------------------------------------------------------------
@@ -880,8 +827,6 @@ hipError_t setup_resident(size_t max_lds)
 }
 
 bool resident_supported(int32_t L) { return L >= 8 && L <= 48 && (L % 8) == 0; }
-
-size_t pt_step_lds(int32_t nrep) { return (size_t)nrep * (4 + 4 + 8 + 40); }
 
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t max_n,
                                  int32_t L, int32_t nt, bool shadow, hipStream_t s)
