@@ -252,6 +252,13 @@ __device__ __forceinline__ NbOff nb_offsets(int c, int nq, int u, int w, const U
     return o;
 }
 
+// float4 load at a 32-bit byte offset from a block-uniform base, so the load
+// can address as SGPR base + VGPR offset (no 64-bit address arithmetic).
+__device__ __forceinline__ float4 ld16(const float4 *base, uint32_t byte_off)
+{
+    return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+
 template <int HX>
 __device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uint32_t &Xm)
 {
@@ -281,7 +288,9 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     x_from<HX>(lds32[own + o.d2a], lds32[own + o.d2b], Xp2, Xm2);
     const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise, <= 4
     const uint32_t Z = ZP + ZM;                         // bytewise, <= 2
-    const float4 kc = cx.kc4[own], kxp = cx.kxp4[own], kxm = cx.kxm4[own], kyp = cx.kyp4[own], kym = cx.kym4[own];
+    const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
+    const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob), kyp = ld16(cx.kyp4, ob),
+                 kym = ld16(cx.kym4, ob);
     const U4 rn = philox4x32_10((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
     const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
     const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
