@@ -44,18 +44,25 @@ def build_batch(J, precision, flags=0):
 
 
 def time_form(J, precision, steps, warmup, kernel_timing):
+    """Timed region: `steps` MEAS cycles replayed as hipGraphs.  With
+    kernel_timing, a second pass of the same length brackets every sweep
+    launch with HIP events on the engine's stream (launched eagerly) for the
+    sweep kernel's average duration."""
     eng = build_batch(J, precision)
     eng.run(J.PHASE_EQUIL, warmup)
     eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
-    eng.set_timing(kernel_timing)
     eng.synchronize()
     t0 = time.perf_counter()
     eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
     eng.synchronize()
     dt = time.perf_counter() - t0
-    sweep_ms, launches = eng.timing(0)
+    sweep_ms, launches = 0.0, 0
+    if kernel_timing:
+        eng.set_timing(True)
+        eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
+        sweep_ms, launches = eng.timing(0)
+        eng.set_timing(False)
     path, bytes_per = eng.sweep_path()
-    eng.set_timing(False)
     fetch = eng.fetch(0)
     eng.close()
     su = float(L * L * L) * NREP * NCONF * PTSWAPEVER * steps

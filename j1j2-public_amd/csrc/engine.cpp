@@ -69,13 +69,15 @@ struct Engine {
     bool observed = false;                     // observables valid for the current state
     std::vector<void *> allocs;
     DeviceState st{};
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // replayed schedules: graph[measure][0] = one cycle, [measure][1] = GRAPH_CYCLES cycles
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     Timer timers[4];
 
     ~Engine()
     {
-        for (auto &g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
+        for (auto &gm : graph)
+            for (auto &g : gm)
+                if (g) (void)hipGraphExecDestroy(g);
         for (auto &t : timers) {
             for (auto &e : t.live) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
             for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -362,12 +364,14 @@ int enqueue_cycle(Engine &E, int mode)
     return 0;
 }
 
-int build_graph(Engine &E, int measure)
+constexpr int GRAPH_CYCLES = 16;
+
+int build_graph(Engine &E, int measure, int big)
 {
-    if (E.graph[measure]) return 0;
-    hipGraph_t g = nullptr;
+    if (E.graph[measure][big]) return 0;
     HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_cycle(E, measure);
+    int rc = 0;
+    for (int i = 0; i < (big ? GRAPH_CYCLES : 1) && rc == 0; ++i) rc = enqueue_cycle(E, measure);
     hipGraph_t cap = nullptr;
     const hipError_t e2 = hipStreamEndCapture(E.stream, &cap);
     if (rc) {
@@ -375,12 +379,11 @@ int build_graph(Engine &E, int measure)
         return rc;
     }
     HIPCHK(e2);
-    g = cap;
     hipGraphExec_t ex = nullptr;
-    const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    const hipError_t e3 = hipGraphInstantiate(&ex, cap, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(cap);
     HIPCHK(e3);
-    E.graph[measure] = ex;
+    E.graph[measure][big] = ex;
     return 0;
 }
 
@@ -396,12 +399,18 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
                 if ((rc = reset_measurement(E, c))) return rc;
     }
     const bool graph = E.use_graph && !E.timing;
-    if (graph && ncycles > 0 && (rc = build_graph(E, measure))) return rc;
-    for (int i = 0; i < ncycles; ++i) {
-        if (graph) HIPCHK(hipGraphLaunch(E.graph[measure], E.stream));
-        else if ((rc = enqueue_cycle(E, measure))) return rc;
-        E.cycles_done += 1;
+    int done = 0;
+    if (graph && ncycles >= GRAPH_CYCLES) {
+        if ((rc = build_graph(E, measure, 1))) return rc;
+        for (; done + GRAPH_CYCLES <= ncycles; done += GRAPH_CYCLES)
+            HIPCHK(hipGraphLaunch(E.graph[measure][1], E.stream));
     }
+    if (graph && done < ncycles && (rc = build_graph(E, measure, 0))) return rc;
+    for (; done < ncycles; ++done) {
+        if (graph) HIPCHK(hipGraphLaunch(E.graph[measure][0], E.stream));
+        else if ((rc = enqueue_cycle(E, measure))) return rc;
+    }
+    E.cycles_done += (uint64_t)ncycles;
     if (E.st.fuse_locnem && ncycles > 0) {
         // accumulate the state the last measuring cycle left behind
         HIPCHK(launch_locnem(E.st, E.max_n, 1, E.stream));
