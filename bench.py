@@ -146,12 +146,23 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=0, help="cycles for the FP64 form (default steps/10)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run 5 plain device copies of this many bytes (PMC counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; the path would shard disorder configs "
               "across GPUs with no exchange during the run (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
     import j1j2_engine as J
+
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     fp32 = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=True)
     res = {
