@@ -1,0 +1,139 @@
+"""Statistical parity on the GPU.
+
+1. Against the unmodified reference (its own runs, kept as fixtures in
+   tests/golden/): the same disorder config and start, the same run lengths;
+   64 independent engine chains (different Philox keys) give the mean and the
+   single-run spread; z = (reference - mean) / sqrt(s^2 + s^2/64), the
+   reference's single run being assigned the engine's single-run spread s.
+2. FP32 batched form against the FP64 form at L = 32, 16 replicas x 64 disorder
+   configs: paired per-config differences, z = mean / (std / sqrt(64)).
+
+Every z-score must satisfy |z| < 4 and at least 75% (resp. 85%) of them |z| < 2.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import j1j2_engine as J
+import oracle as O
+from gpu_cases import say
+from refdata import find, read_table
+
+pytestmark = pytest.mark.gpu
+
+NCHAIN = 64
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+
+# (name, BASEFIELDS index, reference file prefix, column)
+QUANTITIES = [("e", 9, "cfen", 1), ("c", 11, "cfen", 3), ("|m|", 0, "cfma", 1), ("chi", 3, "cfma", 4),
+              ("|str|", 13, "cfst", 1), ("str2", 16, "cfst", 2), ("nem", 22, "cfst", 4),
+              ("|strx|", 14, "cfst", 11), ("|stry|", 15, "cfst", 16), ("nbar", 37, "cfnm", 1),
+              ("nbar2", 39, "cfnm", 7), ("UBN", 41, "cfnm", 10)]
+
+
+def zsummary(zs, label):
+    zs = np.asarray(zs, np.float64)
+    frac2 = float(np.mean(np.abs(zs) < 2.0))
+    say(f"{label}: {zs.size} z-scores, max |z| = {np.max(np.abs(zs)):.2f}, within 2 sigma: {100 * frac2:.0f}%")
+    os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, f"stat_{label}.json"), "w") as f:
+        json.dump({"z": zs.tolist(), "frac_within_2sigma": frac2}, f)
+    return frac2
+
+
+def chains_vs_reference(g, p, spins, variant, golden_dir):
+    g.nconf = NCHAIN
+    eng = J.Engine([g])
+    for c in range(NCHAIN):
+        eng.upload_disorder(c, p)
+        eng.set_spins(c, spins)
+    eng.run(J.PHASE_EQUIL, g.neq)
+    eng.run(J.PHASE_MEAS, g.nmess)
+    bf = np.stack([eng.fetch(c)["basefields"] for c in range(NCHAIN)])      # chain, field, slot
+    outs = [eng.fetch(c) for c in range(NCHAIN)]
+    eng.close()
+    zs, rows = [], []
+    gd = os.path.join(golden_dir, "ref_" + variant)
+    for name, f, pref, col in QUANTITIES:
+        ref = read_table(find(gd, pref))[:, col]
+        mu, s = bf[:, f, :].mean(0), bf[:, f, :].std(0, ddof=1)
+        z = (ref - mu) / np.sqrt(s ** 2 * (1 + 1.0 / NCHAIN) + 1e-300)
+        zs.extend(z.tolist())
+        rows.append((name, ref, mu, z))
+    return zs, rows, outs
+
+
+def test_pt_matches_reference_l8(golden_dir):
+    """Reference run ref_pt_l8_stat: L = 8, NREPLICA = 4, T = 8.0-11.5, STR-X start,
+    NEQ = 2000, NMESS = 40000 cycles, PTSWAPEVER = 5, disorder seed IRINIT = 1."""
+    say("case PT L=8 vs reference (64 chains)")
+    L, R = 8, 4
+    g = J.make_params(L, L, nreplica=R, neq=2000, nmess=40000, seed=777)
+    p = J.make_disorder(L, L, 3.0, 1, True, 2.0)
+    spins = np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R)
+    zs, rows, outs = chains_vs_reference(g, p, spins, "pt_l8_stat", golden_dir)
+    # swap acceptance per pair (cfpt: attempts, accepts)
+    pt = open(find(os.path.join(golden_dir, "ref_pt_l8_stat"), "cfpt")).read().splitlines()[1:R]
+    ref_acc = np.array([float(line.split()[5]) for line in pt])
+    acc = np.stack([o["swap_acc"] / np.maximum(o["swap_att"], 1) for o in outs])
+    att = np.stack([o["swap_att"] for o in outs])
+    assert np.all(att == 21000)                       # (NEQ + NMESS) / 2 per pair, as the reference
+    mu, s = acc.mean(0), acc.std(0, ddof=1)
+    zs.extend(((ref_acc - mu) / np.sqrt(s ** 2 * (1 + 1.0 / NCHAIN))).tolist())
+    for name, ref, mu_, z in rows:
+        say(f"  {name:7s} ref {np.array2string(ref, precision=5)} engine {np.array2string(mu_, precision=5)} "
+            f"z {np.array2string(z, precision=2)}")
+    frac = zsummary(zs, "pt_l8_vs_reference")
+    assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
+
+
+def test_local_matches_reference_l8(golden_dir):
+    """Reference run ref_local_l8: LOCAL, L = 8, T = 10, HOT
+    start, NEQ = NMESS = 1000 sweeps.  The start is the reference's: KISS seeded
+    with IRINIT = 1, 3*L^3 draws consumed by Setup_vacancies (J1J2_6tv.f90:
+    1116-1175), then Initialize_spins."""
+    say("case LOCAL L=8 vs reference (64 chains)")
+    L = 8
+    N = L * L * L
+    u = O.kiss_stream(1, 4 * N)[3 * N:]
+    spins = np.where(u < 0.5, 1, -1).astype(np.int8)
+    g = J.make_params(L, L, nreplica=1, ptswapever=1, sampler=J.SAMPLER_LOCAL, ladder=J.LADDER_LOCAL,
+                      tmin=10.0, tmax=10.0, dt=1.0, neq=1000, nmess=1000, seed=4242)
+    p = J.make_disorder(L, L, 3.0, 1, True, 2.0)
+    zs, rows, _ = chains_vs_reference(g, p, spins, "local_l8", golden_dir)
+    for name, ref, mu_, z in rows:
+        say(f"  {name:7s} ref {np.array2string(ref, precision=5)} engine {np.array2string(mu_, precision=5)} "
+            f"z {np.array2string(z, precision=2)}")
+    frac = zsummary(zs, "local_l8_vs_reference")
+    assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
+
+
+def test_fp32_matches_fp64_l32():
+    """FP32 batched form vs FP64 plain form at L = 32, 16 replicas x 64 configs."""
+    say("case FP32 vs FP64 at L=32")
+    L, R, C = 32, 16, 64
+    res = {}
+    for prec in (J.FP32_BATCHED, J.FP64_PLAIN):
+        g = J.make_params(L, L, nreplica=R, nconf=C, precision=prec, neq=300, nmess=1500, seed=99)
+        eng = J.Engine([g])
+        for c in range(C):
+            eng.upload_disorder(c, J.make_disorder(L, L, 3.0, 1 + c, True, 2.0))
+            eng.set_spins(c, np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R))
+        eng.run(J.PHASE_EQUIL, g.neq)
+        eng.run(J.PHASE_MEAS, g.nmess)
+        outs = [eng.fetch(c) for c in range(C)]
+        res[prec] = (np.stack([o["basefields"] for o in outs]),
+                     np.stack([o["swap_acc"] / o["swap_att"] for o in outs]))
+        eng.close()
+    zs = []
+    for name, f, _, _ in QUANTITIES:
+        d = res[J.FP32_BATCHED][0][:, f, :] - res[J.FP64_PLAIN][0][:, f, :]
+        z = d.mean(0) / (d.std(0, ddof=1) / np.sqrt(C) + 1e-300)
+        zs.extend(z.tolist())
+        say(f"  {name:7s} max |z| over slots {np.max(np.abs(z)):.2f}")
+    d = res[J.FP32_BATCHED][1] - res[J.FP64_PLAIN][1]
+    zs.extend((d.mean(0) / (d.std(0, ddof=1) / np.sqrt(C))).tolist())
+    frac = zsummary(zs, "fp32_vs_fp64_l32")
+    assert np.all(np.abs(zs) < 4.0) and frac >= 0.85
