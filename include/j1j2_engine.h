@@ -101,10 +101,13 @@ int j1j2_make_disorder(int32_t L, int32_t H, double xi, int32_t seed, int32_t ma
                        double phi, double *out);
 
 /* Initialize_spin_array (J1J2_6tv.f90:1179-1209) for one replica: start 0 = HOT
- * (KISS05 seeded with `seed`, i.e. IRINIT+(iconf-1)*NREPLICA+rep as in
- * Initialize_pt_state, J1J2_6tv.f90:1238), 1 = FER, 2 = STR; straxis 0 = X,
+ * (KISS05 seeded with `seed`, then `skip` draws discarded, then one draw per
+ * site: IRINIT+(iconf-1)*NREPLICA+rep and skip 0 for Initialize_pt_state,
+ * J1J2_6tv.f90:1238; IRINIT+iconf-1 and skip 3*L*L*H for the LOCAL start after
+ * Setup_vacancies, J1J2_6tv.f90:316-322), 1 = FER, 2 = STR; straxis 0 = X,
  * 1 = Y.  spins: L*L*H int8, x fastest. */
-int j1j2_initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int8_t *spins);
+int j1j2_initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int64_t skip,
+                       int8_t *spins);
 
 /* Engine lifetime.  j1j2_create = one group.  j1j2_create_batch takes ngroups
  * parameter sets (uneven L / NREPLICA batches) whose ptswapever, ptmeasindex,

@@ -625,11 +625,12 @@ int j1j2_make_disorder(int32_t L, int32_t H, double xi, int32_t seed, int32_t ma
     return corr_field(L, L, H, xi, seed, map_to_uniform != 0, phi, out);
 }
 
-int j1j2_initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int8_t *spins)
+int j1j2_initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int64_t skip,
+                       int8_t *spins)
 {
-    if (!spins || L < 1 || H < 1 || start < 0 || start > 2 || straxis < 0 || straxis > 1)
+    if (!spins || L < 1 || H < 1 || start < 0 || start > 2 || straxis < 0 || straxis > 1 || skip < 0)
         return fail(J1J2_EINVAL, "invalid initial-state request");
-    initial_spins(L, H, start, straxis, seed, spins);
+    initial_spins(L, H, start, straxis, seed, skip, spins);
     return 0;
 }
 

@@ -172,11 +172,12 @@ int corr_field(int32_t n1, int32_t n2, int32_t n3, double xi, int32_t seed, bool
     return 0;
 }
 
-void initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int8_t *s)
+void initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int64_t skip, int8_t *s)
 {
     const int64_t N = (int64_t)L * L * H;
     if (start == 0) {
         Kiss05 rng(seed);
+        for (int64_t i = 0; i < skip; ++i) (void)rng.uniform();
         for (int64_t i = 0; i < N; ++i) s[i] = rng.uniform() < 0.5 ? 1 : -1;
     } else if (start == 1) {
         for (int64_t i = 0; i < N; ++i) s[i] = 1;

@@ -23,7 +23,7 @@ int corr_field(int32_t n1, int32_t n2, int32_t n3, double xi, int32_t seed, bool
                double max_value, double *out);
 
 // Initialize_spin_array (J1J2_6tv.f90:1179-1209); start 0 HOT, 1 FER, 2 STR.
-void initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int8_t *s);
+void initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int64_t skip, int8_t *s);
 
 // Setup_temperature_ladder (J1J2_6tv.f90:928-962).
 void temperature_ladder(int32_t mode, int32_t ntemp, double tmin, double tmax, double dt,

@@ -70,7 +70,7 @@ def _declare(L):
     sig = {
         "j1j2_ladder": [ctypes.POINTER(Params), P],
         "j1j2_make_disorder": [i32, i32, d, i32, i32, d, P],
-        "j1j2_initial_spins": [i32, i32, i32, i32, i32, P],
+        "j1j2_initial_spins": [i32, i32, i32, i32, i32, i64, P],
         "j1j2_create": [ctypes.POINTER(Params), i32, ctypes.POINTER(P)],
         "j1j2_create_batch": [P, i32, i32, ctypes.POINTER(P)],
         "j1j2_last_error": [ctypes.c_char_p, i32],
@@ -136,9 +136,9 @@ def make_disorder(L, H, xi=3.0, seed=1, map_to_uniform=True, phi=2.0):
     return out
 
 
-def initial_spins(L, H, start, straxis=0, seed=1):
+def initial_spins(L, H, start, straxis=0, seed=1, skip=0):
     s = np.zeros(L * L * H, np.int8)
-    check(lib().j1j2_initial_spins(L, H, start, straxis, seed, _p(s)))
+    check(lib().j1j2_initial_spins(L, H, start, straxis, seed, skip, _p(s)))
     return s
 
 

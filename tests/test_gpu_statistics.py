@@ -97,8 +97,8 @@ def test_local_matches_reference_l8(golden_dir):
     say("case LOCAL L=8 vs reference (64 chains)")
     L = 8
     N = L * L * L
-    u = O.kiss_stream(1, 4 * N)[3 * N:]
-    spins = np.where(u < 0.5, 1, -1).astype(np.int8)
+    spins = J.initial_spins(L, L, J.START_HOT, 0, 1, skip=3 * N)
+    assert np.array_equal(spins, np.where(O.kiss_stream(1, 4 * N)[3 * N:] < 0.5, 1, -1))
     g = J.make_params(L, L, nreplica=1, ptswapever=1, sampler=J.SAMPLER_LOCAL, ladder=J.LADDER_LOCAL,
                       tmin=10.0, tmax=10.0, dt=1.0, neq=1000, nmess=1000, seed=4242)
     p = J.make_disorder(L, L, 3.0, 1, True, 2.0)
