@@ -101,36 +101,37 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 }
 
 // ---------------------------------------------------------------------------
-// Acceptance threshold of the FP32 rule: 2^24 * 2^y for y in [-64, 0].
-// y = n + f, n = rint(y), |f| <= 1/2, 2^f by a degree-6 near-minimax
-// polynomial (relative error < 1.6e-8) with coefficients pre-scaled by 2^24.
+// Acceptance threshold of the FP32 rule: 2^32 * 2^y.  y = n + f, n = rint(y),
+// |f| <= 1/2, 2^f by a degree-6 near-minimax polynomial (relative error
+// < 1.6e-8) with coefficients pre-scaled by 2^32.  No clamp: y >= 0 gives
+// >= 2^32 and very negative y gives 0 (v_ldexp_f32 saturates n); the host
+// keeps |y| far below 2^30 (see the temperature check of the FP32 form).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float threshold_clamped(float y)
+__device__ __forceinline__ float threshold32(float y)
 {
     float n = rintf(y);
     float f = y - n;
-    float p = 0x1.41d332p+11f;
-    p = __builtin_fmaf(p, f, 0x1.5f456ap+14f);
-    p = __builtin_fmaf(p, f, 0x1.3b2dbcp+17f);
-    p = __builtin_fmaf(p, f, 0x1.c6aed4p+19f);
-    p = __builtin_fmaf(p, f, 0x1.ebfbdap+21f);
-    p = __builtin_fmaf(p, f, 0x1.62e430p+23f);
-    p = __builtin_fmaf(p, f, 0x1.000000p+24f);
+    float p = 0x1.41d332p+19f;
+    p = __builtin_fmaf(p, f, 0x1.5f456ap+22f);
+    p = __builtin_fmaf(p, f, 0x1.3b2dbcp+25f);
+    p = __builtin_fmaf(p, f, 0x1.c6aed4p+27f);
+    p = __builtin_fmaf(p, f, 0x1.ebfbdap+29f);
+    p = __builtin_fmaf(p, f, 0x1.62e430p+31f);
+    p = __builtin_fmaf(p, f, 0x1.000000p+32f);
     return ldexpf(p, (int)n);
 }
 
-// Metropolis decision of the FP32 rule for the site held in byte j of the
-// spin word `sw` (b = 1 for s = +1), local field h, b2 = 2 beta log2(e), and
-// Philox word r.  y = -b2 s h; a move with y >= 0 (dE <= 0) is always taken,
-// otherwise it is taken iff u24 = r >> 8 < 2^24 2^max(y, -64).  Clamping y to
-// [-64, 0] folds both cases into one comparison: y = 0 gives the threshold
-// 2^24, above every u24.
-__device__ __forceinline__ bool decide_f32_word(float h, uint32_t sw, int j, float b2, uint32_t r)
+// Metropolis decision of the FP32 rule for the site whose spin is bit `bit`
+// of the word `sw` (set for s = +1), local field h, b2 = 2 beta log2(e), and
+// the site's Philox word r: y = -s b2 h, accept iff (float)r <= 2^32 2^y.
+// A move with y >= 0 (dE <= 0) is always taken: its threshold is >= 2^32, the
+// largest value (float)r rounds to.
+__device__ __forceinline__ bool decide_f32_word(float h, uint32_t sw, int bit, float b2, uint32_t r)
 {
     const float g = b2 * h;
-    const uint32_t sgn = (sw << (31 - 8 * j)) & 0x80000000u;
-    const float y = __builtin_amdgcn_fmed3f(__uint_as_float(__float_as_uint(g) ^ sgn), -64.0f, 0.0f);
-    return (float)(r >> 8) < threshold_clamped(y);
+    const uint32_t sgn = (sw << (31 - bit)) & 0x80000000u;
+    const float y = __uint_as_float(__float_as_uint(g) ^ sgn);
+    return (float)r <= threshold32(y);
 }
 
 // ---------------------------------------------------------------------------

@@ -59,6 +59,7 @@ struct Engine {
     std::vector<ConfConst> cconst;
     std::vector<LatDesc> lats;
     std::vector<std::vector<double>> temps;   // per config
+    std::vector<double> hbound;               // per config: bound on |h| of the FP32 rule
     int64_t nsites = 0, nslots = 0, npairs = 0, nhist = 0, nseries = 0, nflips = 0, spin_bytes = 0;
     int32_t max_n = 0;
     int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global
@@ -137,7 +138,7 @@ int ref_site(int L, int x, int y, int z) { return x + L * y + L * L * z; }
 // host-side FP32 constants, fixed-point disorder and FP64 field of one config,
 // written in the internal site order
 void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p, std::vector<float> kd[5],
-                       std::vector<double> &p64, std::vector<int64_t> &q)
+                       std::vector<double> &p64, std::vector<int32_t> &qh, std::vector<int32_t> &ql)
 {
     const int L = cd.L, H = cd.H, N = cd.N;
     std::vector<float> Kx(N), Ky(N);
@@ -150,7 +151,8 @@ void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p,
             }
     for (int P = 0; P < 5; ++P) kd[P].assign(N, 0.0f);
     p64.assign(N, 0.0);
-    q.assign(N, 0);
+    qh.assign(N, 0);
+    ql.assign(N, 0);
     for (int z = 0; z < H; ++z)
         for (int y = 0; y < L; ++y)
             for (int x = 0; x < L; ++x) {
@@ -160,12 +162,15 @@ void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p,
                 const float kyp = Ky[i], kym = Ky[ref_site(L, x, (y + L - 1) % L, z)];
                 const double sK = (((double)kxp + (double)kxm) + (double)kyp) + (double)kym;
                 kd[0][ii] = (float)((-sK - 4.0 * cc.J2) - 2.0 * cc.J3);
-                kd[1][ii] = 2.0f * kxp;
-                kd[2][ii] = 2.0f * kxm;
-                kd[3][ii] = 2.0f * kyp;
-                kd[4][ii] = 2.0f * kym;
+                // 2K scaled by 2^14 (exact): the sweep multiplies them by 2^-14 b
+                kd[1][ii] = std::ldexp(2.0f * kxp, 14);
+                kd[2][ii] = std::ldexp(2.0f * kxm, 14);
+                kd[3][ii] = std::ldexp(2.0f * kyp, 14);
+                kd[4][ii] = std::ldexp(2.0f * kym, 14);
                 p64[ii] = p[i];
-                q[ii] = (int64_t)std::llrint(p[i] * 0x1p32);
+                const int64_t q = (int64_t)std::llrint(p[i] * 0x1p32);     // |q| < 2^38
+                qh[ii] = (int32_t)(q >> 16);
+                ql[ii] = (int32_t)(q & 0xffff);
             }
 }
 
@@ -174,14 +179,16 @@ int upload_site_planes(Engine &E, int conf, const double *p)
     const ConfDesc &cd = E.confs[conf];
     std::vector<float> kd[5];
     std::vector<double> p64;
-    std::vector<int64_t> q;
-    build_site_planes(cd, E.cconst[conf], p, kd, p64, q);
+    std::vector<int32_t> qh, ql;
+    build_site_planes(cd, E.cconst[conf], p, kd, p64, qh, ql);
     for (int P = 0; P < 5; ++P)
         HIPCHK(hipMemcpyAsync(const_cast<float *>(E.st.kd) + P * E.nsites + cd.site_off, kd[P].data(),
                               sizeof(float) * cd.N, hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipMemcpyAsync(const_cast<double *>(E.st.p64) + cd.site_off, p64.data(), sizeof(double) * cd.N,
                           hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipMemcpyAsync(const_cast<int64_t *>(E.st.q) + cd.site_off, q.data(), sizeof(int64_t) * cd.N,
+    HIPCHK(hipMemcpyAsync(const_cast<int32_t *>(E.st.qh) + cd.site_off, qh.data(), sizeof(int32_t) * cd.N,
+                          hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(const_cast<int32_t *>(E.st.ql) + cd.site_off, ql.data(), sizeof(int32_t) * cd.N,
                           hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
     return 0;
@@ -392,6 +399,13 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
     if (phase != J1J2_PHASE_EQUIL && phase != J1J2_PHASE_MEAS) return fail(J1J2_EINVAL, "unknown phase");
     const int measure = phase == J1J2_PHASE_MEAS ? 1 : 0;
     int rc;
+    // FP32 rule: |y| = |b2 h| must stay far inside the int range of rint(y)
+    // (threshold32 has no clamp); 2^24 leaves every threshold exactly 0 or inf.
+    if (E.common.precision == J1J2_FP32_BATCHED)
+        for (int c = 0; c < (int)E.confs.size(); ++c)
+            for (double t : E.temps[c])
+                if (2.0 * LOG2E / t * E.hbound[c] >= 0x1p24)
+                    return fail(J1J2_EINVAL, "temperature too low for the FP32 form (|beta dE| >= 2^24 ln 2)");
     if (begin) {
         HIPCHK(launch_begin_phase(E.st, E.stream));
         if (measure)
@@ -538,6 +552,8 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
             E->confs.push_back(cd);
             E->cconst.push_back(cc);
             E->temps.push_back(t);
+            E->hbound.push_back(12.0 * std::max(std::fabs(cc.J1h), std::fabs(cc.J1v)) + 12.0 * std::fabs(cc.J2) +
+                                6.0 * std::fabs(cc.J3) + 1.0);
         }
         E->groups.push_back(G);
     }
@@ -552,10 +568,10 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     DeviceState &st = E->st;
     const int64_t nconf = (int64_t)E->confs.size();
     LatDesc *lats; ConfDesc *confs; ConfConst *cconst;
-    float *kd, *b2; double *p64, *beta; int64_t *q;
+    float *kd, *b2; double *p64, *beta; int32_t *q;
     if ((rc = E->alloc(lats, nlat)) || (rc = E->alloc(confs, nconf)) || (rc = E->alloc(cconst, nconf)) ||
         (rc = E->alloc(st.spins, E->spin_bytes)) || (rc = E->alloc(kd, 5 * E->nsites)) ||
-        (rc = E->alloc(p64, E->nsites)) || (rc = E->alloc(q, E->nsites)) ||
+        (rc = E->alloc(p64, E->nsites)) || (rc = E->alloc(q, 2 * E->nsites)) ||
         (rc = E->alloc(st.obs, nlat * OBS_N)) || (rc = E->alloc(st.rep_at_temp, E->nslots)) ||
         (rc = E->alloc(st.temp_of_rep, E->nslots)) || (rc = E->alloc(beta, E->nslots)) ||
         (rc = E->alloc(b2, E->nslots)) || (rc = E->alloc(st.clk, 1)) ||
@@ -575,7 +591,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
     st.lats = lats; st.confs = confs; st.cconst = cconst;
     st.nlat = (int32_t)nlat; st.nconf = (int32_t)nconf; st.nsites = E->nsites;
-    st.kd = kd; st.p64 = p64; st.q = q; st.beta = beta; st.b2 = b2;
+    st.kd = kd; st.p64 = p64; st.qh = q; st.ql = q + E->nsites; st.beta = beta; st.b2 = b2;
     st.series_cap = series_cap;
     st.ptswapever = c0.ptswapever; st.ptmeasindex = c0.ptmeasindex; st.bins = c0.bins;
     st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;
@@ -670,8 +686,15 @@ int j1j2_upload_disorder(void *handle, int32_t conf, const double *p, int64_t n)
     if ((rc = check_conf(E, conf))) return rc;
     const ConfDesc &cd = E->confs[conf];
     if (!p || n != cd.N) return fail(J1J2_EINVAL, "disorder array must hold L*L*H values");
-    for (int64_t i = 0; i < n; ++i)
+    double pmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
         if (!(std::fabs(p[i]) < 64.0)) return fail(J1J2_EINVAL, "disorder values must be finite with |p| < 64");
+        pmax = std::max(pmax, std::fabs(p[i]));
+    }
+    // |h| <= |C| + 2 sum|K| + 8|J2| + 4|J3| with |C| <= sum|K| + 4|J2| + 2|J3|, |K| <= |J1| + 2 pmax
+    const ConfConst &cc = E->cconst[conf];
+    const double kmax = std::max(std::fabs(cc.J1h), std::fabs(cc.J1v)) + 2.0 * pmax;
+    E->hbound[conf] = 12.0 * kmax + 12.0 * std::fabs(cc.J2) + 6.0 * std::fabs(cc.J3) + 1.0;
     HIPCHK(hipSetDevice(E->device));
     E->observed = false;
     return upload_site_planes(*E, conf, p);

@@ -24,9 +24,9 @@ struct DeviceState {
     int64_t nsites;            // sum over configs of N (per-site planes)
     // lattice state
     uint8_t *spins;            // sum over lattices of N bytes, internal layout
-    const float *kd;           // 5 planes x nsites: kc, 2Kx+, 2Kx-, 2Ky+, 2Ky-
+    const float *kd;           // 5 planes x nsites: kc, then 2^14 * (2Kx+, 2Kx-, 2Ky+, 2Ky-)
     const double *p64;         // nsites
-    const int64_t *q;          // nsites, llrint(p * 2^32)
+    const int32_t *qh, *ql;    // nsites each: llrint(p * 2^32) = qh * 2^16 + ql, 0 <= ql < 2^16
     int64_t *obs;              // nlat x OBS_N
     // ladder
     int32_t *rep_at_temp;      // nslots (per config blocks of nrep)

@@ -8,7 +8,7 @@
 //   * the c-th colour's sites are numbered q = (z*L/2 + y/2)*L/2 + x/2; the
 //     uniform for site q in sweep S of replica r of config iconf is word q&3
 //     of Philox4x32-10(counter = (q/4, c | r<<2, S lo, S hi),
-//     key = (seed, iconf)), u24 = word >> 8;
+//     key = (seed, iconf)); the site uses the whole 32-bit word;
 //   * FP32 form: h = C_i + sum_j 2K_j b_j + 2 J2 D + 2 J3 Z (b in {0,1});
 //     FP64 form: Compute_local_field (J1J2_6tv.f90:2498-2541) in its order;
 //     Metropolis acceptance of Metro_sweep (J1J2_6tv.f90:2572-2604).
@@ -22,14 +22,15 @@ namespace {
 
 constexpr int GLB_THREADS = 256;
 
-__device__ __forceinline__ uint32_t byte_bit(uint32_t w, int j) { return (w >> (8 * j)) & 1u; }
+__device__ __forceinline__ uint32_t lds_bit(uint32_t w, int j) { return (w >> (8 * j + 2)) & 1u; }
 __device__ __forceinline__ float byte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xffu); }
 
 
-// FP64 decision of Metro_sweep (J1J2_6tv.f90:2572-2604) with the engine's uniform
+// FP64 decision of Metro_sweep (J1J2_6tv.f90:2572-2604) with the engine's
+// uniform u = r 2^-32 from the site's Philox word
 __device__ __forceinline__ bool decide_f64(double field, bool up, double beta, uint32_t r)
 {
-    double u = (double)(r >> 8) * 0x1p-24;
+    double u = (double)r * 0x1p-32;
     if (up) {
         if (field <= 0.0) return true;
         return u < exp64(-((2.0 * field) * beta));
@@ -138,9 +139,10 @@ __device__ __forceinline__ void x_pair(const uint32_t *base, int uoff, int w, ui
 }
 
 // Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) over the units of
-// one lattice in the colour-plane layout (sp32 in LDS or global memory):
-// l_i = 2 (a(+y) + a(-y) - a(+x) - a(-x)), a = 1 if the bond is anti-aligned.
-template <int WPH, int HX>
+// one lattice in the colour-plane layout: l_i = 2 (a(+y) + a(-y) - a(+x) -
+// a(-x)), a = 1 if the bond is anti-aligned.  A site's spin is bit SH of its
+// byte: SH = 0 in global memory, 2 in the resident sweep's LDS copy.
+template <int WPH, int HX, int SH>
 __device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
                                              int32_t *locspin, int32_t *locnem)
 {
@@ -158,29 +160,39 @@ __device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H,
         int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            lsp[j] += (int)((S >> (8 * j)) & 1u) * 2 - 1;
-            lnp[j] += 2 * ((int)((ay >> (8 * j)) & 0xffu) - (int)((ax >> (8 * j)) & 0xffu));
+            lsp[j] += (int)((S >> (8 * j + SH)) & 1u) * 2 - 1;
+            lnp[j] += 2 * ((int)((ay >> (8 * j + SH)) & 0xffu) - (int)((ax >> (8 * j + SH)) & 0xffu));
         }
         reinterpret_cast<int4 *>(locspin)[dw] = ls;
         reinterpret_cast<int4 *>(locnem)[dw] = lv;
     }
 }
 
-template <int WPH>
+template <int WPH, int SH>
 __device__ __forceinline__ void locnem_units(const uint32_t *sp32, int H, int t0, int stride, int32_t *locspin,
                                              int32_t *locnem)
 {
     for (int c = 0; c < 4; ++c) {
-        if (c & 1) locnem_plane<WPH, 1>(sp32, c, H, t0, stride, locspin, locnem);
-        else locnem_plane<WPH, 0>(sp32, c, H, t0, stride, locspin, locnem);
+        if (c & 1) locnem_plane<WPH, 1, SH>(sp32, c, H, t0, stride, locspin, locnem);
+        else locnem_plane<WPH, 0, SH>(sp32, c, H, t0, stride, locspin, locnem);
     }
 }
 
-// Integer observables (see k_observables) of one colour plane, forward bonds,
-// one dword (4 sites) per step.
+// Per-thread partial sums of the integer observables in int32 (a thread covers
+// at most a few hundred sites), except the high part of the field sum.
+struct ObsAcc {
+    int32_t m, sx, sy, bx, by, bd, bz, eql;
+    int64_t eqh;
+};
+
+// Integer observables (see k_observables) of one colour plane of the resident
+// sweep's LDS copy (spin = bit 2 of its byte), forward bonds, one dword
+// (4 sites) per step.  The field term sum q_i l_i is split as q = qh 2^16 + ql
+// (|qh| < 2^22, 0 <= ql < 2^16, |l| <= 4) so that every product is a 24-bit
+// multiply: sum qh l per dword fits int32 and is widened, sum ql l stays int32.
 template <int WPH, int HX>
 __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
-                                              const int64_t *qb, int64_t (&v)[OBS_N])
+                                              const int32_t *qh, const int32_t *ql, ObsAcc &a)
 {
     const int nq = 4 * WPH * WPH * H;
     for (int t = t0; t < nq; t += stride) {
@@ -194,20 +206,25 @@ __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H
         x_pair<WPH, HX>(sp32, (c ^ 1) * nq + u * WPH, w, Xp0, Xm0);
         x_pair<WPH, HX>(sp32, (c ^ 3) * nq + g.uyp * WPH, w, Xp1, Xm1);
         const int m4 = 2 * __builtin_popcount(S) - 4;
-        v[OBS_M] += m4;
-        v[OBS_SX] += HX ? -m4 : m4;
-        v[OBS_SY] += g.p ? -m4 : m4;
-        v[OBS_BX] += 4 - 2 * __builtin_popcount(S ^ Xp0);
-        v[OBS_BY] += 4 - 2 * __builtin_popcount(S ^ YP);
-        v[OBS_BZ] += 4 - 2 * __builtin_popcount(S ^ ZP);
-        v[OBS_BD] += 8 - 2 * (__builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1));
-        const uint32_t ax = (S ^ Xp0) + (S ^ Xm0), ay = (S ^ YP) + (S ^ YM);
-        const int64_t *qw = qb + 4 * (int64_t)(c * nq + t);
+        a.m += m4;
+        a.sx += HX ? -m4 : m4;
+        a.sy += g.p ? -m4 : m4;
+        a.bx += 4 - 2 * __builtin_popcount(S ^ Xp0);
+        a.by += 4 - 2 * __builtin_popcount(S ^ YP);
+        a.bz += 4 - 2 * __builtin_popcount(S ^ ZP);
+        a.bd += 8 - 2 * (__builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1));
+        const uint32_t ax = (S ^ Xp0) + (S ^ Xm0), ay = (S ^ YP) + (S ^ YM);   // bytes 0, 4, 8
+        const int dw = c * nq + t;
+        const int4 h4 = reinterpret_cast<const int4 *>(qh)[dw], l4 = reinterpret_cast<const int4 *>(ql)[dw];
+        const int hq[4] = {h4.x, h4.y, h4.z, h4.w}, lq[4] = {l4.x, l4.y, l4.z, l4.w};
+        int32_t sh = 0;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int l = 2 * ((int)((ay >> (8 * j)) & 0xff) - (int)((ax >> (8 * j)) & 0xff));
-            v[OBS_EQ] += qw[j] * (int64_t)l;
+            const int l = 2 * ((int)((ay >> (8 * j + 2)) & 0xffu) - (int)((ax >> (8 * j + 2)) & 0xffu));
+            sh += __mul24(hq[j], l);
+            a.eql += __mul24(lq[j], l);
         }
+        a.eqh += sh;
     }
 }
 
@@ -221,7 +238,7 @@ __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H
 struct SweepCtx {
     const float4 *kc4, *kxp4, *kxm4, *kyp4, *kym4;
     const double *p64;     // config's FP64 field (shadow evaluation)
-    float b2, j2x2, j3x2;
+    float b2, j2q, j3q;        // j2q = 2 J2 / 4, j3q = 2 J3 / 4 (exact scalings)
     double beta;
     uint32_t ctr1, sweep_lo, sweep_hi, key0, key1;
 };
@@ -271,6 +288,21 @@ __device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uin
     }
 }
 
+// In the resident sweep's LDS copy a site's byte is 4b (spin bit at bit 2),
+// which is the high byte of the f16 value 2^-14 b.  With the coupling planes
+// stored as 2^14 * 2K, fma(2^-14 b, 2^14 2K, h) has the exact product b 2K, so
+// it rounds exactly as fma(b, 2K, h) and the compiler issues it as one
+// v_fma_mix_f32 reading the f16 half directly (no byte-to-float conversion).
+// f16_odd keeps bytes 1 and 3 in place (sites 1, 3: low and high half);
+// f16_even moves bytes 0 and 2 up into the high bytes of the halves.
+__device__ __forceinline__ uint32_t f16_odd(uint32_t w) { return w & 0xFF00FF00u; }
+__device__ __forceinline__ uint32_t f16_even(uint32_t w) { return __builtin_amdgcn_perm(w, w, 0x020c000cu); }
+__device__ __forceinline__ float f16_site(uint32_t even, uint32_t odd, int j)
+{
+    const uint32_t w = (j & 1) ? odd : even;
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)((j & 2) ? (w >> 16) : w));
+}
+
 // Update the 4 sites of colour-plane dword `own` (plane-relative index t =
 // u*WPH + w, i.e. colour indices 4t..4t+3, Philox group t).
 template <int WPH, int HX, bool SHADOW>
@@ -286,8 +318,10 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     x_from<HX>(lds32[own + o.xa], lds32[own + o.xb], Xp0, Xm0);
     x_from<HX>(lds32[own + o.d1a], lds32[own + o.d1b], Xp1, Xm1);
     x_from<HX>(lds32[own + o.d2a], lds32[own + o.d2b], Xp2, Xm2);
-    const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise, <= 4
-    const uint32_t Z = ZP + ZM;                         // bytewise, <= 2
+    const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise 4 * (0..4)
+    const uint32_t Z = ZP + ZM;                         // bytewise 4 * (0..2)
+    const uint32_t xpe = f16_even(Xp0), xpo = f16_odd(Xp0), xme = f16_even(Xm0), xmo = f16_odd(Xm0);
+    const uint32_t ype = f16_even(YP), ypo = f16_odd(YP), yme = f16_even(YM), ymo = f16_odd(YM);
     const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
     const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob), kyp = ld16(cx.kyp4, ob),
                  kym = ld16(cx.kym4, ob);
@@ -302,22 +336,22 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         float h = kca[j];
-        h = __builtin_fmaf(byte_f(Xp0, j), kxpa[j], h);
-        h = __builtin_fmaf(byte_f(Xm0, j), kxma[j], h);
-        h = __builtin_fmaf(byte_f(YP, j), kypa[j], h);
-        h = __builtin_fmaf(byte_f(YM, j), kyma[j], h);
-        h = __builtin_fmaf(byte_f(D, j), cx.j2x2, h);
-        h = __builtin_fmaf(byte_f(Z, j), cx.j3x2, h);
-        const bool acc = decide_f32_word(h, S, j, cx.b2, ra[j]);
+        h = __builtin_fmaf(f16_site(xpe, xpo, j), kxpa[j], h);
+        h = __builtin_fmaf(f16_site(xme, xmo, j), kxma[j], h);
+        h = __builtin_fmaf(f16_site(ype, ypo, j), kypa[j], h);
+        h = __builtin_fmaf(f16_site(yme, ymo, j), kyma[j], h);
+        h = __builtin_fmaf(byte_f(D, j), cx.j2q, h);          // (4D) (2 J2 / 4)
+        h = __builtin_fmaf(byte_f(Z, j), cx.j3q, h);
+        const bool acc = decide_f32_word(h, S, 8 * j + 2, cx.b2, ra[j]);
         if constexpr (SHADOW) {
-            const bool up = byte_bit(S, j) != 0;
+            const bool up = lds_bit(S, j) != 0;
             const int m = 4 * w + j;                          // position in the half-row
             const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
             const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
-            const double sv[10] = {pm1(byte_bit(YP, j)), pm1(byte_bit(YM, j)), pm1(byte_bit(Xp0, j)),
-                                   pm1(byte_bit(Xm0, j)), pm1(byte_bit(Xp1, j)), pm1(byte_bit(Xm1, j)),
-                                   pm1(byte_bit(Xp2, j)), pm1(byte_bit(Xm2, j)), pm1(byte_bit(ZP, j)),
-                                   pm1(byte_bit(ZM, j))};
+            const double sv[10] = {pm1(lds_bit(YP, j)), pm1(lds_bit(YM, j)), pm1(lds_bit(Xp0, j)),
+                                   pm1(lds_bit(Xm0, j)), pm1(lds_bit(Xp1, j)), pm1(lds_bit(Xm1, j)),
+                                   pm1(lds_bit(Xp2, j)), pm1(lds_bit(Xm2, j)), pm1(lds_bit(ZP, j)),
+                                   pm1(lds_bit(ZM, j))};
             const int xbase = 4 * (own + o.xa - w);           // first site of the other half-row
             const double pv[5] = {cx.p64[4 * own + j], cx.p64[4 * (own + o.yp) + j], cx.p64[4 * (own + o.ym) + j],
                                   cx.p64[xbase + mo_p], cx.p64[xbase + mo_m]};
@@ -325,7 +359,7 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
             sh_prop += 1;
             sh_mis += (a64 != acc);
         }
-        flip |= (acc ? 1u : 0u) << (8 * j);
+        flip |= (acc ? 4u : 0u) << (8 * j);
     }
     lds32[own] = S ^ flip;
 }
@@ -340,18 +374,30 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
     if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) {
         // Each thread keeps its (y/2, w) and steps z by NT/PER_Z (even, so y's
         // parity is fixed): every neighbour offset is constant except at the
-        // z boundary.
-        constexpr int ZSTEP = NT / PER_Z;
-        const int tz = tid % PER_Z, w = tz % WPH, z0 = tid / PER_Z;
+        // z boundary.  Two steps per trip: the second step's LDS addresses are
+        // the first's plus a constant, which the reads take as an immediate.
+        constexpr int ZSTEP = NT / PER_Z, STEP = ZSTEP * PER_Z;
+        const int tz = tid % PER_Z, w = tz % WPH;
+        int z0 = tid / PER_Z;
+        if constexpr (PER_Z % 64 == 0) z0 = __builtin_amdgcn_readfirstlane(z0);   // wave-uniform
         const int u0 = z0 * HALF + tz / WPH;
         const UnitGeo g0 = unit_geo<WPH>(u0, c, H);
         NbOff o = nb_offsets<WPH, HX>(c, nq, u0, w, g0);
         const int dY = ((c ^ 2) - c) * nq;
-        for (int z = z0; z < H; z += ZSTEP) {
-            const int t = z * PER_Z + tz;
-            o.zp = dY + (z == H - 1 ? -(H - 1) * PER_Z : PER_Z);
-            o.zm = dY + (z == 0 ? (H - 1) * PER_Z : -PER_Z);
-            update_word<WPH, HX, SHADOW>(lds32, cx, cc, c * nq + t, t, w, o, sh_prop, sh_mis);
+        const int nk = (H - z0 + ZSTEP - 1) / ZSTEP;
+        const int own0 = c * nq + z0 * PER_Z + tz;
+        for (int k = 0; k < nk; k += 2) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int kk = k + i;
+                if (kk < nk) {
+                    const int z = z0 + kk * ZSTEP;
+                    o.zp = dY + (z == H - 1 ? -(H - 1) * PER_Z : PER_Z);
+                    o.zm = dY + (z == 0 ? (H - 1) * PER_Z : -PER_Z);
+                    update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, o, sh_prop,
+                                                 sh_mis);
+                }
+            }
         }
     } else {
         for (int t = tid; t < nq; t += NT) {
@@ -372,8 +418,10 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
-    for (int i = tid; i < N / 16; i += NT)
-        reinterpret_cast<uint4 *>(lds32)[i] = reinterpret_cast<const uint4 *>(gsp)[i];
+    for (int i = tid; i < N / 16; i += NT) {            // bytes b -> 4b in LDS
+        const uint4 v = reinterpret_cast<const uint4 *>(gsp)[i];
+        reinterpret_cast<uint4 *>(lds32)[i] = make_uint4(v.x << 2, v.y << 2, v.z << 2, v.w << 2);
+    }
 
     const ConfDesc cd = st.confs[d.conf];
     const ConfConst cc = st.cconst[d.conf];
@@ -386,8 +434,8 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.kym4 = reinterpret_cast<const float4 *>(st.kd + 4 * st.nsites + d.site_off);
     cx.p64 = st.p64 + d.site_off;
     cx.b2 = st.b2[d.slot_base + slot];
-    cx.j2x2 = cc.j2x2;
-    cx.j3x2 = cc.j3x2;
+    cx.j2q = cc.j2x2 * 0.25f;
+    cx.j3q = cc.j3x2 * 0.25f;
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
@@ -397,7 +445,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     // the previous cycle measured and left this replica at slot PTMEASINDEX:
     // accumulate its local nematic sums from the state it ended in
     if (st.fuse_locnem && slot == st.ptmeasindex - 1 && st.locnem_pending[d.conf]) {
-        locnem_units<WPH>(lds32, H, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
+        locnem_units<WPH, 2>(lds32, H, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
         if (tid == 0) st.locsamples[d.conf] += 1;
         __syncthreads();   // every read of the unswept state precedes the first colour pass
     }
@@ -416,12 +464,12 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
 
     // ---- observables of the final state (integer sums; phi part fixed point)
-    int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int64_t *qb = st.q + d.site_off;
+    ObsAcc a = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 4; ++c) {
-        if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, qb, v);
-        else observe_plane<WPH, 0>(lds32, c, H, tid, NT, qb, v);
+        if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
+        else observe_plane<WPH, 0>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
     }
+    int64_t v[OBS_N] = {a.m, a.sx, a.sy, a.bx, a.by, a.bd, a.bz, a.eqh * 65536 + a.eql};
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
         sh_mis = wave_sum<unsigned long long>(sh_mis);
@@ -430,8 +478,10 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
             atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
         }
     }
-    for (int i = tid; i < N / 16; i += NT)
-        reinterpret_cast<uint4 *>(gsp)[i] = reinterpret_cast<const uint4 *>(lds32)[i];
+    for (int i = tid; i < N / 16; i += NT) {
+        const uint4 v = reinterpret_cast<const uint4 *>(lds32)[i];
+        reinterpret_cast<uint4 *>(gsp)[i] = make_uint4(v.x >> 2, v.y >> 2, v.z >> 2, v.w >> 2);
+    }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
@@ -452,12 +502,14 @@ template <int WPH, int NT, bool SHADOW>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : NT / 64, 8)))
 void k_sweep_resident(DeviceState st, int32_t lat_begin, int32_t nlat, int32_t per_xcd)
 {
+    // the lattice at LDS address 0 (so neighbour addresses need no base), the
+    // reduction scratch right after it
     extern __shared__ uint32_t lds32[];
-    __shared__ int64_t red[NT / 64][OBS_N];
     const int li = xcd_lattice(per_xcd);
     if (li >= nlat) return;
     const int lat = lat_begin + li;
-    resident_body<WPH, NT, SHADOW>(st, st.lats[lat], lat, lds32, red);
+    const LatDesc d = st.lats[lat];
+    resident_body<WPH, NT, SHADOW>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4));
 }
 
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
@@ -466,11 +518,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 :
 void k_sweep_resident_mixed(DeviceState st, int32_t lat_begin, int32_t nlat, int32_t per_xcd)
 {
     extern __shared__ uint32_t lds32[];
-    __shared__ int64_t red[NT / 64][OBS_N];
     const int li = xcd_lattice(per_xcd);
     if (li >= nlat) return;
     const int lat = lat_begin + li;
     const LatDesc d = st.lats[lat];
+    int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4);
     switch (d.L) {
         case 8: resident_body<1, NT, SHADOW>(st, d, lat, lds32, red); break;
         case 16: resident_body<2, NT, SHADOW>(st, d, lat, lds32, red); break;
@@ -553,11 +605,12 @@ __global__ __launch_bounds__(GLB_THREADS) void k_sweep_global(DeviceState st, in
         const uint32_t up = sp[nb.i];
         bool acc, other = false;
         if (PREC == 1 || SHADOW) {
+            // planes hold 2^14 * 2K: (2^-14 b) (2^14 2K) is exactly b 2K
             float h = kc[nb.i];
-            h = __builtin_fmaf((float)b[2], kxp[nb.i], h);
-            h = __builtin_fmaf((float)b[3], kxm[nb.i], h);
-            h = __builtin_fmaf((float)b[0], kyp[nb.i], h);
-            h = __builtin_fmaf((float)b[1], kym[nb.i], h);
+            h = __builtin_fmaf((float)b[2] * 0x1p-14f, kxp[nb.i], h);
+            h = __builtin_fmaf((float)b[3] * 0x1p-14f, kxm[nb.i], h);
+            h = __builtin_fmaf((float)b[0] * 0x1p-14f, kyp[nb.i], h);
+            h = __builtin_fmaf((float)b[1] * 0x1p-14f, kym[nb.i], h);
             h = __builtin_fmaf((float)(b[4] + b[5] + b[6] + b[7]), cc.j2x2, h);
             h = __builtin_fmaf((float)(b[8] + b[9]), cc.j3x2, h);
             const bool a32 = decide_f32_word(h, up, 0, b2, ra[j]);
@@ -599,7 +652,7 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
     const LatDesc d = st.lats[lat];
     const int L = d.L, H = d.H, N = d.N;
     const uint8_t *sp = st.spins + d.spin_off;
-    const int64_t *qb = st.q + d.site_off;
+    const int32_t *qh = st.qh + d.site_off, *ql = st.ql + d.site_off;
     int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < N; i += OBS_THREADS) {
         int x, y, z;
@@ -616,7 +669,7 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
         v[OBS_BY] += si * s[0];
         v[OBS_BD] += si * s[4] + si * s[5];
         v[OBS_BZ] += si * s[8];
-        v[OBS_EQ] += qb[i] * (int64_t)(si * (s[2] + s[3] - s[0] - s[1]));
+        v[OBS_EQ] += ((int64_t)qh[i] * 65536 + ql[i]) * (int64_t)(si * (s[2] + s[3] - s[0] - s[1]));
     }
     block_reduce_obs<OBS_THREADS>(v, red);
     if (threadIdx.x == 0) {
@@ -765,12 +818,12 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
     int32_t *ls = st.locspin + d.site_off, *ln = st.locnem + d.site_off;
     const int t0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
     switch (L) {
-        case 8: locnem_units<1>(sp32, H, t0, stride, ls, ln); break;
-        case 16: locnem_units<2>(sp32, H, t0, stride, ls, ln); break;
-        case 24: locnem_units<3>(sp32, H, t0, stride, ls, ln); break;
-        case 32: locnem_units<4>(sp32, H, t0, stride, ls, ln); break;
-        case 40: locnem_units<5>(sp32, H, t0, stride, ls, ln); break;
-        case 48: locnem_units<6>(sp32, H, t0, stride, ls, ln); break;
+        case 8: locnem_units<1, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 16: locnem_units<2, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 24: locnem_units<3, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 32: locnem_units<4, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 40: locnem_units<5, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 48: locnem_units<6, 0>(sp32, H, t0, stride, ls, ln); break;
         default:
             for (int i = t0; i < N; i += stride) {
                 int x, y, z;
@@ -788,6 +841,7 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
 
 __global__ void k_begin_phase(DeviceClock *clk) { clk->phase_start = clk->cycle; }
 
+// reduction scratch after the lattice in the resident sweep's dynamic LDS
 constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (512 / 64) * OBS_N;
 
 }  // namespace
@@ -829,7 +883,8 @@ hipError_t setup_resident(size_t max_lds)
         for (int nt : {256, 512})
             for (bool sh : {false, true}) {
                 hipError_t e = hipFuncSetAttribute((const void *)pick_resident(L, nt, sh),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)(max_lds + RES_STATIC_LDS));
                 if (e != hipSuccess) return e;
             }
     return hipSuccess;
@@ -843,7 +898,7 @@ hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32
     if (nlat <= 0) return hipSuccess;
     const int per_xcd = (nlat + 7) / 8;
     hipLaunchKernelGGL(pick_resident(L, nt, shadow), dim3(8 * per_xcd), dim3(nt == 512 ? 512 : 256),
-                       (size_t)max_n, s, st, lat_begin, nlat, per_xcd);
+                       (size_t)max_n + RES_STATIC_LDS, s, st, lat_begin, nlat, per_xcd);
     return hipGetLastError();
 }
 

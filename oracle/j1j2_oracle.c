@@ -113,13 +113,15 @@ void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * ====================================================================== */
 float ora_accept_threshold_f32(float y)
 {
-    /* 2^24 * 2^y.  y = n + f, n = rint(y), |f| <= 1/2; 2^f by a degree-6
+    /* 2^32 * 2^y.  y = n + f, n = rint(y), |f| <= 1/2; 2^f by a degree-6
      * polynomial (near-minimax on [-1/2, 1/2], relative error < 1.6e-8),
-     * coefficients pre-scaled by 2^24 (exact power-of-two scaling). */
-    const float c0 = 0x1.000000p+24f, c1 = 0x1.62e430p+23f, c2 = 0x1.ebfbdap+21f,
-                c3 = 0x1.c6aed4p+19f, c4 = 0x1.3b2dbcp+17f, c5 = 0x1.5f456ap+14f,
-                c6 = 0x1.41d332p+11f;
-    y = fmaxf(y, -64.0f);
+     * coefficients pre-scaled by 2^32 (exact power-of-two scaling).  No
+     * clamp: for y >= 0 the result is >= 2^32, for y <= -183 it is 0 (the
+     * engine's v_ldexp_f32 and ldexpf agree; n is bounded here only to keep
+     * the int conversion defined, which leaves ldexp's 0 / inf unchanged). */
+    const float c0 = 0x1.000000p+32f, c1 = 0x1.62e430p+31f, c2 = 0x1.ebfbdap+29f,
+                c3 = 0x1.c6aed4p+27f, c4 = 0x1.3b2dbcp+25f, c5 = 0x1.5f456ap+22f,
+                c6 = 0x1.41d332p+19f;
     float n = rintf(y);
     float f = y - n;
     float p = c6;
@@ -129,6 +131,8 @@ float ora_accept_threshold_f32(float y)
     p = fmaf(p, f, c2);
     p = fmaf(p, f, c1);
     p = fmaf(p, f, c0);
+    if (n < -1000.0f) n = -1000.0f;
+    if (n > 1000.0f) n = 1000.0f;
     return ldexpf(p, (int)n);
 }
 
@@ -288,21 +292,23 @@ float ora_field_f32(const ora_lattice *lat, const int8_t *s, int32_t site)
     return h;
 }
 
-/* Decision rules.  u24 = top 24 bits of the Philox word. */
+/* Decision rules.  r = the site's 32-bit Philox word.
+ * FP32: y = -s (b2 h) with b2 = (float)(2 beta log2 e); accept iff
+ * (float)r <= 2^32 2^y.  Every dE <= 0 move is taken (y >= 0 gives a
+ * threshold >= 2^32, the largest value (float)r can round to). */
 static int decide_f32(const ora_lattice *lat, const int8_t *s, int32_t site, float b2, uint32_t r)
 {
     float h = ora_field_f32(lat, s, site);
-    float sh = s[site] > 0 ? h : -h;
-    if (!(sh > 0.0f)) return 1;
-    float y = -(b2 * sh);
-    return (float)(r >> 8) < ora_accept_threshold_f32(y);
+    float g = b2 * h;
+    float y = s[site] > 0 ? -g : g;
+    return (float)r <= ora_accept_threshold_f32(y);
 }
 
 static int decide_f64(const ora_lattice *lat, const int8_t *s, int32_t site, double beta, uint32_t r)
 {
     /* Metro_sweep acceptance, J.f90:2572-2604 */
     double field = ora_field_f64(lat, s, site, NULL);
-    double u = (double)(r >> 8) * 0x1p-24;
+    double u = (double)r * 0x1p-32;
     if (s[site] == 1) {
         if (field <= 0.0) return 1;
         return u < ora_exp64(-((2.0 * field) * beta));

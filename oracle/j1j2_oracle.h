@@ -49,7 +49,7 @@ void   ora_kiss_fill(int32_t seed, int64_t n, double *out);   /* n draws after k
 void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
 /* ---------------- deterministic exponentials ---------------- */
-/* 2^24 * 2^y for y >= -64 (y is clamped below at -64); FP32, fma Horner. */
+/* 2^32 * 2^y in FP32 (fma Horner, no clamp; 0 below y = -183, inf above 96). */
 float  ora_accept_threshold_f32(float y);
 /* e^x in FP64: Cody-Waite reduction + degree-13 Taylor Horner, fma. */
 double ora_exp64(double x);

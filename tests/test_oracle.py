@@ -39,10 +39,15 @@ def test_exp64_accuracy():
 def test_accept_threshold_accuracy():
     ys = np.linspace(-64, 0, 100001).astype(np.float32)
     for y in ys[::7]:
-        ref = 2.0 ** (24.0 + float(y))
+        ref = 2.0 ** (32.0 + float(y))
         assert abs(O.accept_threshold_f32(float(y)) - ref) <= 1.2e-7 * ref
-    assert O.accept_threshold_f32(0.0) == 2.0 ** 24
-    assert O.accept_threshold_f32(-1000.0) == 2.0 ** -40
+    # every dE <= 0 move is taken: (float)r <= 2^32 for every 32-bit r
+    assert O.accept_threshold_f32(0.0) == 2.0 ** 32
+    assert O.accept_threshold_f32(-0.0) == 2.0 ** 32
+    assert all(O.accept_threshold_f32(y) >= 2.0 ** 32 for y in (1e-7, 0.3, 0.5, 0.51, 7.0, 95.0))
+    assert O.accept_threshold_f32(200.0) == math.inf
+    assert O.accept_threshold_f32(-200.0) == 0.0
+    assert O.accept_threshold_f32(-5000.0) == 0.0
 
 
 # ------------------------------------------------------------------ reference fixtures
