@@ -100,6 +100,52 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
     return U4{c0, c1, c2, c3};
 }
 
+// The same generator when only c0 varies across the lanes (c1, c2, c3 and the
+// key are wave-uniform, as in the sweeps): the first three rounds are written
+// so that every uniform sub-expression is formed once on the scalar unit and
+// each vector xor takes a single scalar operand.
+__device__ __forceinline__ U4 philox4x32_10_u(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                               uint32_t k0, uint32_t k1)
+{
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u, M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    // round 1: p0 vector, p1 uniform
+    uint64_t p0 = (uint64_t)M0 * c0;
+    uint64_t p1 = (uint64_t)M1 * c2;
+    uint32_t a0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;                  // uniform
+    uint32_t a2 = (uint32_t)(p0 >> 32) ^ (c3 ^ k1);                // vector
+    uint32_t a1 = (uint32_t)p1, a3 = (uint32_t)p0;                 // uniform, vector
+    k0 += W0; k1 += W1;
+    // round 2: p0 uniform, p1 vector
+    p0 = (uint64_t)M0 * a0;
+    p1 = (uint64_t)M1 * a2;
+    uint32_t b0 = (uint32_t)(p1 >> 32) ^ (a1 ^ k0);                // vector
+    uint32_t b2 = a3 ^ ((uint32_t)(p0 >> 32) ^ k1);                // vector
+    uint32_t b1 = (uint32_t)p1, b3 = (uint32_t)p0;                 // vector, uniform
+    k0 += W0; k1 += W1;
+    // round 3: c3 uniform
+    p0 = (uint64_t)M0 * b0;
+    p1 = (uint64_t)M1 * b2;
+    c0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), b1, k0, 0x96);
+    c2 = (uint32_t)(p0 >> 32) ^ (b3 ^ k1);
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    k0 += W0; k1 += W1;
+#pragma unroll
+    for (int r = 3; r < 10; ++r) {
+        p0 = (uint64_t)M0 * c0;
+        p1 = (uint64_t)M1 * c2;
+        const uint32_t n0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c1, k0, 0x96);
+        const uint32_t n2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c3, k1, 0x96);
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += W0;
+        k1 += W1;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
 // ---------------------------------------------------------------------------
 // Acceptance threshold of the FP32 rule: 2^32 * 2^y.  y = n + f, n = rint(y),
 // |f| <= 1/2, 2^f by a degree-6 near-minimax polynomial (relative error

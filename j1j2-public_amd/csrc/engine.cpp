@@ -181,6 +181,11 @@ int upload_site_planes(Engine &E, int conf, const double *p)
     std::vector<double> p64;
     std::vector<int32_t> qh, ql;
     build_site_planes(cd, E.cconst[conf], p, kd, p64, qh, ql);
+    int64_t qsum = 0;
+    for (int i = 0; i < cd.N; ++i) qsum += (int64_t)qh[i] * 65536 + ql[i];
+    E.cconst[conf].qsum = qsum;
+    HIPCHK(hipMemcpyAsync(const_cast<ConfConst *>(E.st.cconst) + conf, &E.cconst[conf], sizeof(ConfConst),
+                          hipMemcpyHostToDevice, E.stream));
     for (int P = 0; P < 5; ++P)
         HIPCHK(hipMemcpyAsync(const_cast<float *>(E.st.kd) + P * E.nsites + cd.site_off, kd[P].data(),
                               sizeof(float) * cd.N, hipMemcpyHostToDevice, E.stream));

@@ -13,6 +13,7 @@ namespace j1j2 {
 struct ConfConst {
     double J1h, J1v, J2, J3;
     float j2x2, j3x2;          // (float)(2*J2), (float)(2*J3)
+    int64_t qsum;              // sum over the sites of llrint(p * 2^32)
 };
 
 struct DeviceState {

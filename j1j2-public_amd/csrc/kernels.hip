@@ -288,6 +288,72 @@ __device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uin
     }
 }
 
+// The same observables when a plane's words map onto the threads with a fixed
+// (y/2, w) per thread and z stepping by an even NT/PER_Z (L = 8, 16, 32 with
+// 256 or 512 threads): neighbour offsets are then constant per thread, the
+// y parity too, so the bond counts accumulate raw popcounts (turned into
+// sums of s_i s_j once at the end) and the field term accumulates
+// q_i e_i with e_i = l_i / 2 + 2 in 0..4 (sum q_i l_i = 2 sum q_i e_i -
+// 4 sum q_i, the last sum being a per-config constant).
+struct ObsFast {
+    uint32_t nw;                       // words visited
+    uint32_t ps, px, py, pz, pd;       // popcounts of S, S^X+, S^Y+, S^Z+, S^D(+x+y) + S^D(-x+y)
+    int32_t sxm, sym;                  // signed sums of 2 popc(S) - 4 (x and y stripe)
+    int64_t eh;                        // sum qh e
+    int32_t el;                        // sum ql e
+};
+
+template <int WPH, int NT, int HX>
+__device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const int32_t *qh,
+                                             const int32_t *ql, ObsFast &a)
+{
+    constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = NT / PER_Z, STEP = ZSTEP * PER_Z;
+    const int nq = PER_Z * H, tid = threadIdx.x;
+    const int tz = tid % PER_Z, w = tz % WPH;
+    int z0 = tid / PER_Z;
+    if constexpr (PER_Z % 64 == 0) z0 = __builtin_amdgcn_readfirstlane(z0);
+    const int u0 = z0 * HALF + tz / WPH;
+    const UnitGeo g = unit_geo<WPH>(u0, c, H);
+    const NbOff o = nb_offsets<WPH, HX>(c, nq, u0, w, g);
+    const int dY = ((c ^ 2) - c) * nq;
+    const int own0 = c * nq + z0 * PER_Z + tz;
+    uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0;
+    int64_t eh = 0;
+    int32_t el = 0;
+    for (int z = z0; z < H; z += ZSTEP) {
+        const int own = own0 + (z - z0) * PER_Z;
+        const uint32_t S = lds32[own];
+        const uint32_t YP = lds32[own + o.yp], YM = lds32[own + o.ym];
+        const uint32_t ZP = lds32[own + dY + (z == H - 1 ? -(H - 1) * PER_Z : PER_Z)];
+        uint32_t Xp0, Xm0, Xp1, Xm1;
+        x_from<HX>(lds32[own + o.xa], lds32[own + o.xb], Xp0, Xm0);
+        x_from<HX>(lds32[own + o.d1a], lds32[own + o.d1b], Xp1, Xm1);
+        const uint32_t sx0 = S ^ Xp0, sy0 = S ^ YP;
+        ps += __builtin_popcount(S);
+        px += __builtin_popcount(sx0);
+        py += __builtin_popcount(sy0);
+        pz += __builtin_popcount(S ^ ZP);
+        pd += __builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1);
+        nw += 1;
+        const uint32_t ax = sx0 + (S ^ Xm0), ay = sy0 + (S ^ YM);          // bytes 0, 4, 8
+        const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;                 // bytes l/2 + 2
+        const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
+        const int4 h4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(qh) + ob);
+        const int4 l4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(ql) + ob);
+        const int32_t sh = __mul24(h4.x, (int)(e & 0xffu)) + __mul24(h4.y, (int)((e >> 8) & 0xffu)) +
+                           __mul24(h4.z, (int)((e >> 16) & 0xffu)) + __mul24(h4.w, (int)(e >> 24));
+        el += __mul24(l4.x, (int)(e & 0xffu)) + __mul24(l4.y, (int)((e >> 8) & 0xffu)) +
+              __mul24(l4.z, (int)((e >> 16) & 0xffu)) + __mul24(l4.w, (int)(e >> 24));
+        eh += sh;
+    }
+    const int32_t m = 2 * (int32_t)ps - 4 * (int32_t)nw;
+    a.ps += ps; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += nw;
+    a.sxm += HX ? -m : m;
+    a.sym += g.p ? -m : m;
+    a.eh += eh;
+    a.el += el;
+}
+
 // In the resident sweep's LDS copy a site's byte is 4b (spin bit at bit 2),
 // which is the high byte of the f16 value 2^-14 b.  With the coupling planes
 // stored as 2^14 * 2K, fma(2^-14 b, 2^14 2K, h) has the exact product b 2K, so
@@ -303,21 +369,57 @@ __device__ __forceinline__ float f16_site(uint32_t even, uint32_t odd, int j)
     return (float)__builtin_bit_cast(_Float16, (uint16_t)((j & 2) ? (w >> 16) : w));
 }
 
+// Hide how an LDS address was formed from the optimiser: it stays one
+// register (an empty asm statement; no instruction is emitted).
+template <typename T>
+__device__ __forceinline__ void opaque(T *&p)
+{
+    auto a = (__attribute__((address_space(3))) T *)p;
+    asm volatile("" : "+v"(a));
+    p = (T *)a;
+}
+
+// LDS addresses of a word and of its neighbour words.
+struct NbPtr {
+    uint32_t *own;
+    const uint32_t *yp, *ym, *zp, *zm, *xa, *xb, *d1a, *d1b, *d2a, *d2b;
+};
+
+__device__ __forceinline__ NbPtr nb_ptrs(uint32_t *lds32, int own, const NbOff &o)
+{
+    NbPtr P;
+    P.own = lds32 + own;
+    P.yp = P.own + o.yp; P.ym = P.own + o.ym; P.zp = P.own + o.zp; P.zm = P.own + o.zm;
+    P.xa = P.own + o.xa; P.xb = P.own + o.xb; P.d1a = P.own + o.d1a; P.d1b = P.own + o.d1b;
+    P.d2a = P.own + o.d2a; P.d2b = P.own + o.d2b;
+    return P;
+}
+
+__device__ __forceinline__ NbPtr nb_shift(const NbPtr &P, int d)
+{
+    NbPtr Q;
+    Q.own = P.own + d;
+    Q.yp = P.yp + d; Q.ym = P.ym + d; Q.zp = P.zp + d; Q.zm = P.zm + d; Q.xa = P.xa + d; Q.xb = P.xb + d;
+    Q.d1a = P.d1a + d; Q.d1b = P.d1b + d; Q.d2a = P.d2a + d; Q.d2b = P.d2b + d;
+    return Q;
+}
+
 // Update the 4 sites of colour-plane dword `own` (plane-relative index t =
-// u*WPH + w, i.e. colour indices 4t..4t+3, Philox group t).
+// u*WPH + w, i.e. colour indices 4t..4t+3, Philox group t); P holds the LDS
+// addresses of the word and its neighbours.
 template <int WPH, int HX, bool SHADOW>
 __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int own,
-                                           int t, int w, const NbOff &o, unsigned long long &sh_prop,
+                                           int t, int w, const NbPtr &P, unsigned long long &sh_prop,
                                            unsigned long long &sh_mis)
 {
     constexpr int HALF = 4 * WPH;
-    const uint32_t S = lds32[own];
-    const uint32_t YP = lds32[own + o.yp], YM = lds32[own + o.ym];
-    const uint32_t ZP = lds32[own + o.zp], ZM = lds32[own + o.zm];
+    const uint32_t S = *P.own;
+    const uint32_t YP = *P.yp, YM = *P.ym;
+    const uint32_t ZP = *P.zp, ZM = *P.zm;
     uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
-    x_from<HX>(lds32[own + o.xa], lds32[own + o.xb], Xp0, Xm0);
-    x_from<HX>(lds32[own + o.d1a], lds32[own + o.d1b], Xp1, Xm1);
-    x_from<HX>(lds32[own + o.d2a], lds32[own + o.d2b], Xp2, Xm2);
+    x_from<HX>(*P.xa, *P.xb, Xp0, Xm0);
+    x_from<HX>(*P.d1a, *P.d1b, Xp1, Xm1);
+    x_from<HX>(*P.d2a, *P.d2b, Xp2, Xm2);
     const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise 4 * (0..4)
     const uint32_t Z = ZP + ZM;                         // bytewise 4 * (0..2)
     const uint32_t xpe = f16_even(Xp0), xpo = f16_odd(Xp0), xme = f16_even(Xm0), xmo = f16_odd(Xm0);
@@ -325,7 +427,7 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
     const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob), kyp = ld16(cx.kyp4, ob),
                  kym = ld16(cx.kym4, ob);
-    const U4 rn = philox4x32_10((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+    const U4 rn = philox4x32_10_u((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
     const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
     const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
     const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
@@ -352,16 +454,16 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
                                    pm1(lds_bit(Xm0, j)), pm1(lds_bit(Xp1, j)), pm1(lds_bit(Xm1, j)),
                                    pm1(lds_bit(Xp2, j)), pm1(lds_bit(Xm2, j)), pm1(lds_bit(ZP, j)),
                                    pm1(lds_bit(ZM, j))};
-            const int xbase = 4 * (own + o.xa - w);           // first site of the other half-row
-            const double pv[5] = {cx.p64[4 * own + j], cx.p64[4 * (own + o.yp) + j], cx.p64[4 * (own + o.ym) + j],
-                                  cx.p64[xbase + mo_p], cx.p64[xbase + mo_m]};
+            const int xbase = 4 * ((int)(P.xa - lds32) - w);  // first site of the other half-row
+            const double pv[5] = {cx.p64[4 * own + j], cx.p64[4 * (int)(P.yp - lds32) + j],
+                                  cx.p64[4 * (int)(P.ym - lds32) + j], cx.p64[xbase + mo_p], cx.p64[xbase + mo_m]};
             const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[j]);
             sh_prop += 1;
             sh_mis += (a64 != acc);
         }
         flip |= (acc ? 4u : 0u) << (8 * j);
     }
-    lds32[own] = S ^ flip;
+    *P.own = S ^ flip;
 }
 
 // One colour pass over plane c.
@@ -386,25 +488,35 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
         const int dY = ((c ^ 2) - c) * nq;
         const int nk = (H - z0 + ZSTEP - 1) / ZSTEP;
         const int own0 = c * nq + z0 * PER_Z + tz;
+        o.zp = dY + PER_Z;                                // away from the z boundary
+        o.zm = dY - PER_Z;
+        NbPtr P = nb_ptrs(lds32, own0, o);
+        // materialise each address in one VGPR (otherwise the plane base is
+        // re-added to every address on every trip)
+        opaque(P.own); opaque(P.yp); opaque(P.ym); opaque(P.zp); opaque(P.zm); opaque(P.xa); opaque(P.xb);
+        opaque(P.d1a); opaque(P.d1b); opaque(P.d2a); opaque(P.d2b);
         for (int k = 0; k < nk; k += 2) {
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int kk = k + i;
                 if (kk < nk) {
                     const int z = z0 + kk * ZSTEP;
-                    o.zp = dY + (z == H - 1 ? -(H - 1) * PER_Z : PER_Z);
-                    o.zm = dY + (z == 0 ? (H - 1) * PER_Z : -PER_Z);
-                    update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, o, sh_prop,
+                    NbPtr Q = nb_shift(P, i * STEP);
+                    if (z == H - 1) Q.zp -= H * PER_Z;     // periodic in z
+                    if (z == 0) Q.zm += H * PER_Z;
+                    update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q, sh_prop,
                                                  sh_mis);
                 }
             }
+            P = nb_shift(P, 2 * STEP);
         }
     } else {
         for (int t = tid; t < nq; t += NT) {
             const int u = t / WPH, w = t - u * WPH;
             const UnitGeo g = unit_geo<WPH>(u, c, H);
             const NbOff o = nb_offsets<WPH, HX>(c, nq, u, w, g);
-            update_word<WPH, HX, SHADOW>(lds32, cx, cc, c * nq + t, t, w, o, sh_prop, sh_mis);
+            update_word<WPH, HX, SHADOW>(lds32, cx, cc, c * nq + t, t, w, nb_ptrs(lds32, c * nq + t, o), sh_prop,
+                                         sh_mis);
         }
     }
 }
@@ -464,12 +576,32 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
 
     // ---- observables of the final state (integer sums; phi part fixed point)
-    ObsAcc a = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < 4; ++c) {
-        if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
-        else observe_plane<WPH, 0>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
+    int64_t v[OBS_N];
+    constexpr int PER_Z = 4 * WPH * WPH;
+    if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) {
+        ObsFast a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 4; ++c) {
+            if (c & 1) observe_fast<WPH, NT, 1>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
+            else observe_fast<WPH, NT, 0>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
+        }
+        const int32_t nw = (int32_t)a.nw;
+        v[OBS_M] = 2 * (int64_t)a.ps - 4 * (int64_t)nw;
+        v[OBS_SX] = a.sxm;
+        v[OBS_SY] = a.sym;
+        v[OBS_BX] = 4 * (int64_t)nw - 2 * (int64_t)a.px;
+        v[OBS_BY] = 4 * (int64_t)nw - 2 * (int64_t)a.py;
+        v[OBS_BD] = 8 * (int64_t)nw - 2 * (int64_t)a.pd;
+        v[OBS_BZ] = 4 * (int64_t)nw - 2 * (int64_t)a.pz;
+        v[OBS_EQ] = 2 * (a.eh * 65536 + a.el);        // the -4 sum q term is added after the reduction
+    } else {
+        ObsAcc a = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < 4; ++c) {
+            if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
+            else observe_plane<WPH, 0>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
+        }
+        v[OBS_M] = a.m; v[OBS_SX] = a.sx; v[OBS_SY] = a.sy; v[OBS_BX] = a.bx; v[OBS_BY] = a.by;
+        v[OBS_BD] = a.bd; v[OBS_BZ] = a.bz; v[OBS_EQ] = a.eqh * 65536 + a.eql;
     }
-    int64_t v[OBS_N] = {a.m, a.sx, a.sy, a.bx, a.by, a.bd, a.bz, a.eqh * 65536 + a.eql};
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
         sh_mis = wave_sum<unsigned long long>(sh_mis);
@@ -484,6 +616,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
+        if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) v[OBS_EQ] -= 4 * cc.qsum;
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
         for (int k = 0; k < OBS_N; ++k) o[k] = v[k];
