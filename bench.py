@@ -63,11 +63,12 @@ def time_form(J, precision, steps, warmup, kernel_timing):
         sweep_ms, launches = eng.timing(0)
         eng.set_timing(False)
     path, bytes_per = eng.sweep_path()
+    plan = eng.launch_plan()
     fetch = eng.fetch(0)
     eng.close()
     su = float(L * L * L) * NREP * NCONF * PTSWAPEVER * steps
     return dict(seconds=dt, su=su, rate=su / dt, sweep_ms=sweep_ms, launches=launches, path=path,
-                bytes_per=bytes_per, e_slot0=float(fetch["basefields"][9][0]))
+                bytes_per=bytes_per, e_slot0=float(fetch["basefields"][9][0]), plan=plan)
 
 
 def mismatch_rate(J, cycles):
@@ -138,6 +139,99 @@ def cpu_baseline():
                 kind="port", sample=f"oracle FP32 rule, one config, {n} PT cycles, single thread")
 
 
+# ---------------------------------------------------------------- other workloads
+# Reported beside the headline line (python bench.py --workload c4|c5|c2); the
+# default run measures the L=32 batch above only.
+def build_groups(J, groups, precision, flags=0, seed=SEED):
+    """groups: list of (L, nrep, nconf, start) -> one engine over all of them."""
+    import numpy as np
+    params = [J.make_params(Lg, Lg, nreplica=R, nconf=C, iconf0=1 + sum(g[2] for g in groups[:k]),
+                            ptswapever=PTSWAPEVER, precision=precision, flags=flags, tmin=8.0, tmax=11.5,
+                            seed=seed)
+              for k, (Lg, R, C, _) in enumerate(groups)]
+    eng = J.Engine(params)
+    conf = 0
+    for (Lg, R, C, start), g in zip(groups, params):
+        for c in range(C):
+            iconf = g.iconf0 + c
+            eng.upload_disorder(conf, J.make_disorder(Lg, Lg, 3.0, iconf, True, 2.0))
+            st = start if start != "mixed" else ("hot" if c % 2 == 0 else "str")
+            if st == "hot":
+                spins = np.concatenate([J.initial_spins(Lg, Lg, J.START_HOT, 0, (iconf - 1) * R + r + 1)
+                                        for r in range(R)])
+            else:
+                spins = np.tile(J.initial_spins(Lg, Lg, J.START_STR, 0, 0), R)
+            eng.set_spins(conf, spins)
+            conf += 1
+    return eng
+
+
+def time_groups(J, groups, precision, steps, warmup, flags=0):
+    eng = build_groups(J, groups, precision, flags)
+    eng.run(J.PHASE_EQUIL, warmup)
+    eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    eng.set_timing(True)
+    eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
+    sweep_ms, launches = eng.timing(0)
+    eng.set_timing(False)
+    path, bytes_per = eng.sweep_path()
+    plan = eng.launch_plan()
+    eng.close()
+    su = sum(float(Lg ** 3) * R * C for Lg, R, C, _ in groups) * PTSWAPEVER * steps
+    return dict(seconds=dt, su=su, rate=su / dt, sweep_ms=sweep_ms, launches=launches, path=path,
+                bytes_per=bytes_per, plan=plan)
+
+
+def group_mismatch(J, groups, cycles):
+    eng = build_groups(J, groups, J.FP32_BATCHED, J.FLAG_SHADOW_FP64)
+    eng.run(J.PHASE_EQUIL, cycles)
+    tot = mis = 0
+    for c in range(eng.nconf):
+        a, b = eng.mismatch(c)
+        tot += a
+        mis += b
+    eng.close()
+    return mis / max(tot, 1), tot
+
+
+WORKLOADS = {
+    "c4": ("uneven batch: L in {16,24,32,48} x 16 configs each, replicas {8,12,16,24}, "
+           "HOT/STR starts 50/50, T=8.0-11.5, FP32",
+           [(16, 8, 16, "mixed"), (24, 12, 16, "mixed"), (32, 16, 16, "mixed"), (48, 24, 16, "mixed")]),
+    "c5": ("large lattice: L=96, 24 replicas x 8 configs, STR start, T=8.0-11.5, FP32",
+           [(96, 24, 8, "str")]),
+    "c2": ("PT, L=16, 16 replicas x 1 config, STR start, T=8.0-11.5",
+           [(16, 16, 1, "str")]),
+}
+
+
+def other_workload(J, name, steps, warmup):
+    desc, groups = WORKLOADS[name]
+    fp32 = time_groups(J, groups, J.FP32_BATCHED, steps, warmup)
+    n64 = max(steps // 10, 5)
+    fp64 = time_groups(J, groups, J.FP64_PLAIN, n64, max(warmup // 10, 2))
+    su_launch = fp32["su"] / max(fp32["launches"], 1)
+    avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)
+    rate, n = group_mismatch(J, groups, 4)
+    return {
+        "metric": "site updates/s (all replicas)", "value": fp32["rate"], "unit": "site updates/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * fp32["seconds"] / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic", "config": {"workload": desc, "sweep_path": J.PATH_NAMES[fp32["path"]],
+                                        "launch_plan": fp32["plan"]},
+        "sweep_kernel": {"avg_launch_ms": avg_s * 1e3, "launches_per_step": fp32["launches"] / steps,
+                         "algorithmic_GBps": su_launch * fp32["bytes_per"] / avg_s / 1e9,
+                         "frac_of_hbm_peak": su_launch * fp32["bytes_per"] / avg_s / HBM_PEAK},
+        "fp64_plain": {"value": fp64["rate"], "steps": n64, "fp32_over_fp64": fp32["rate"] / fp64["rate"]},
+        "flip_mismatch_rate": {"fp32_vs_fp64": rate, "proposals": n},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +240,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=0, help="cycles for the FP64 form (default steps/10)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", default="c3", choices=["c3"] + sorted(WORKLOADS),
+                    help="c3 = the headline L=32 batch; the others are reported beside it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run 5 plain device copies of this many bytes (PMC counter calibration)")
     args = ap.parse_args()
@@ -164,6 +260,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.workload != "c3":
+        print(json.dumps(other_workload(J, args.workload, args.steps, args.warmup)), flush=True)
+        return
+
     fp32 = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=True)
     res = {
         "metric": "site updates/s (all replicas) at L=32, FP32 vs FP64 form; % HBM peak",
@@ -181,7 +281,7 @@ def main():
         "config": {"workload": "batched PT, L=32, 16 replicas (T=8.0-11.5) x 64 disorder configs "
                                "(PHI=2, XI=3), 5 sweeps + swap + measurement per step",
                    "L": L, "H": L, "nreplica": NREP, "nconf": NCONF, "ptswapever": PTSWAPEVER,
-                   "sweep_path": J.PATH_NAMES[fp32["path"]]},
+                   "sweep_path": J.PATH_NAMES[fp32["path"]], "launch_plan": fp32["plan"]},
     }
     su_launch = fp32["su"] / max(fp32["launches"], 1)
     avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)

@@ -178,6 +178,13 @@ int  j1j2_get_timing(void *handle, int32_t kind, double *total_ms, int64_t *laun
 /* Which sweep path the engine picked (0 = FP64 plain, 1 = FP32 LDS-resident,
  * 2 = FP32 global) and the algorithmic bytes per site update it is priced at. */
 int  j1j2_sweep_path(void *handle, int32_t *path, double *bytes_per_update);
+/* The resident sweep's launch plan as text: per launch the lattice size, the
+ * threads per block and the lattice count, whether the launches run on
+ * parallel graph branches, and whether the autotuner picked it.  The engine
+ * tunes on the first call of j1j2_run / j1j2_run_cycles long enough for the
+ * candidates (at most 16 cycles, counted in the call's cycles); the result of
+ * a run does not depend on the plan.  No counterpart in the reference. */
+int  j1j2_launch_plan(void *handle, char *buf, int32_t len);
 int  j1j2_synchronize(void *handle);
 
 #ifdef __cplusplus

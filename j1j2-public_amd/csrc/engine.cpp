@@ -7,9 +7,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,6 +53,33 @@ struct Timer {
     int64_t launches = 0;
 };
 
+// One launch of the LDS-resident sweep: the blocks of work-list segment
+// [ord0, ord0 + nord), lattices of common size L (0: mixed sizes), nt threads
+// per block, nlat lattices.
+struct ResLaunch {
+    int32_t ord0, nord, L, max_n, nt, nlat;
+};
+
+// Work-list segment of a lattice range (one group, or all lattices).
+struct OrderSeg {
+    int32_t ord0 = 0, nord = 0;
+};
+
+// How a cycle's resident sweeps are launched.  The result of a cycle does not
+// depend on the plan (site updates are keyed by lattice, colour and word, not
+// by thread), so the autotuner may time candidates on real cycles.
+struct Plan {
+    std::vector<ResLaunch> launches;
+    bool concurrent = false;   // one parallel graph branch (side stream) per launch
+    std::string describe() const
+    {
+        std::string d = concurrent ? "concurrent" : "sequential";
+        for (const ResLaunch &r : launches)
+            d += " [L" + std::to_string(r.L) + " nt" + std::to_string(r.nt) + " x" + std::to_string(r.nlat) + "]";
+        return d;
+    }
+};
+
 struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -64,8 +94,16 @@ struct Engine {
     int32_t max_n = 0;
     int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global
     bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
-    int32_t res_nt = 256;                      // threads per block of the resident sweep
+    int32_t res_nt = 256;                      // threads per block of the resident sweep (default plan)
     int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
+    Plan plan;                                 // resident launches of a cycle
+    std::vector<OrderSeg> group_seg;           // work-list segment of each resident group
+    OrderSeg all_seg;                          // of all lattices (when all are resident)
+    bool tuned = false, autotune = true;
+    std::string tune_key;
+    std::vector<hipStream_t> side;             // one per concurrent launch
+    hipEvent_t fork_ev = nullptr;
+    std::vector<hipEvent_t> join_ev;
     uint64_t cycles_done = 0;
     bool observed = false;                     // observables valid for the current state
     std::vector<void *> allocs;
@@ -84,6 +122,9 @@ struct Engine {
             for (auto &e : t.pool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         }
         for (void *p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        for (hipStream_t q : side) (void)hipStreamDestroy(q);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -316,30 +357,54 @@ int collect_timers(Engine &E)
 
 struct SweepArg { int g; int k; int c; };
 
-int enqueue_cycle(Engine &E, int mode)
+// Events the autotuner records around the resident part of one cycle: per
+// launch (sequential plans) and at the end of the whole resident phase.
+struct Probe {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> per;
+    hipEvent_t phase_end = nullptr;
+};
+
+int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe = nullptr)
 {
     // mode bit 0: measure; bit 1: values only (no sweeps, no swaps)
     const bool measure = mode & 1, values_only = mode & 2;
     int rc;
     if (!values_only) {
-        if (E.all_resident) {
-            auto fn = [](Engine &e, const void *) {
-                return launch_sweep_resident(e.st, 0, e.st.nlat, e.max_n, e.uniform_L, e.res_nt, e.shadow,
-                                             e.stream);
+        const Plan &P = plan ? *plan : E.plan;
+        if (P.concurrent && P.launches.size() > 1) {
+            // fork: every launch on its own branch, join before the swap step
+            auto fn = [](Engine &e, const void *a) {
+                const Plan &pl = *static_cast<const Plan *>(a);
+                hipError_t r = hipEventRecord(e.fork_ev, e.stream);
+                for (size_t i = 0; i < pl.launches.size() && r == hipSuccess; ++i) {
+                    const ResLaunch &L = pl.launches[i];
+                    if ((r = hipStreamWaitEvent(e.side[i], e.fork_ev, 0)) != hipSuccess) break;
+                    if ((r = launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.nt, e.shadow, e.side[i])) !=
+                        hipSuccess)
+                        break;
+                    if ((r = hipEventRecord(e.join_ev[i], e.side[i])) != hipSuccess) break;
+                    r = hipStreamWaitEvent(e.stream, e.join_ev[i], 0);
+                }
+                return r;
             };
-            if ((rc = timed(E, 0, fn, nullptr))) return rc;
+            if ((rc = timed(E, 0, fn, &P))) return rc;
         } else {
+            for (const ResLaunch &L : P.launches) {
+                auto fn = [](Engine &e, const void *a) {
+                    const ResLaunch &r = *static_cast<const ResLaunch *>(a);
+                    return launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.nt, e.shadow, e.stream);
+                };
+                const size_t i = (size_t)(&L - P.launches.data());
+                if (probe && i < probe->per.size()) HIPCHK(hipEventRecord(probe->per[i].first, E.stream));
+                if ((rc = timed(E, 0, fn, &L))) return rc;
+                if (probe && i < probe->per.size()) HIPCHK(hipEventRecord(probe->per[i].second, E.stream));
+            }
+        }
+        if (probe && probe->phase_end) HIPCHK(hipEventRecord(probe->phase_end, E.stream));
+        {
             for (int g = 0; g < (int)E.groups.size(); ++g) {
                 const Group &G = E.groups[g];
-                if (G.resident) {
-                    auto fn = [](Engine &e, const void *a) {
-                        const Group &gg = e.groups[*static_cast<const int *>(a)];
-                        return launch_sweep_resident(e.st, gg.lat0, gg.nlat, gg.N, gg.p.L, e.res_nt, e.shadow,
-                                                     e.stream);
-                    };
-                    if ((rc = timed(E, 0, fn, &g))) return rc;
-                    continue;
-                }
+                if (G.resident) continue;
                 for (int k = 0; k < E.common.ptswapever; ++k)
                     for (int c = 0; c < 4; ++c) {
                         SweepArg a{g, k, c};
@@ -399,6 +464,242 @@ int build_graph(Engine &E, int measure, int big)
     return 0;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Picks, per engine shape, the threads per block of each resident launch and
+// whether the lattices of different sizes run as one launch or as parallel
+// per-group launches.  Candidates are timed on real cycles of the run (the
+// result does not depend on the plan) and the choice is cached per process
+// by (device architecture, groups' L, H, lattice count, shadow).
+std::mutex &tune_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+std::map<std::string, Plan> &tune_cache()
+{
+    static std::map<std::string, Plan> m;
+    return m;
+}
+
+ResLaunch group_launch(const Engine &E, int g, int32_t nt)
+{
+    const Group &G = E.groups[g];
+    return {E.group_seg[g].ord0, E.group_seg[g].nord, G.p.L, G.N, nt, G.nlat};
+}
+
+ResLaunch all_launch(const Engine &E, int32_t nt)
+{
+    return {E.all_seg.ord0, E.all_seg.nord, E.uniform_L, E.max_n, nt, E.st.nlat};
+}
+
+Plan default_plan(const Engine &E)
+{
+    Plan P;
+    if (E.all_resident) {
+        P.launches.push_back(all_launch(E, E.res_nt));
+        return P;
+    }
+    for (int g = 0; g < (int)E.groups.size(); ++g)
+        if (E.groups[g].resident) P.launches.push_back(group_launch(E, g, E.res_nt));
+    return P;
+}
+
+// Work list of the lattices of configs [c0, c1): configs largest first,
+// each config's replicas together, configs dealt to the 8 XCDs by least
+// accumulated work, then interleaved so that block 8k + x is XCD x's k-th
+// lattice (-1 pads the shorter XCD lists).
+std::vector<int32_t> make_order(const Engine &E, int c0, int c1)
+{
+    std::vector<int> cs;
+    for (int c = c0; c < c1; ++c) cs.push_back(c);
+    std::stable_sort(cs.begin(), cs.end(), [&](int a, int b) { return E.confs[a].N > E.confs[b].N; });
+    std::vector<std::vector<int32_t>> bins(8);
+    std::vector<int64_t> work(8, 0);
+    for (int c : cs) {
+        const int x = (int)(std::min_element(work.begin(), work.end()) - work.begin());
+        const ConfDesc &cd = E.confs[c];
+        for (int r = 0; r < cd.nrep; ++r) bins[x].push_back(cd.lat0 + r);
+        work[x] += (int64_t)cd.N * cd.nrep;
+    }
+    size_t len = 0;
+    for (auto &b : bins) len = std::max(len, b.size());
+    std::vector<int32_t> out(8 * len, -1);
+    for (size_t k = 0; k < len; ++k)
+        for (int x = 0; x < 8; ++x)
+            if (k < bins[x].size()) out[8 * k + x] = bins[x][k];
+    return out;
+}
+
+int ensure_side(Engine &E, size_t n)
+{
+    if (!E.fork_ev) HIPCHK(hipEventCreateWithFlags(&E.fork_ev, hipEventDisableTiming));
+    while (E.side.size() < n) {
+        hipStream_t q = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        E.side.push_back(q);
+        hipEvent_t ev = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        E.join_ev.push_back(ev);
+    }
+    return 0;
+}
+
+void drop_graphs(Engine &E)
+{
+    for (auto &gm : E.graph)
+        for (auto &g : gm)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+}
+
+// One eager cycle under plan P; per-launch times (sequential plans) and the
+// resident phase's time, in ms.
+int tune_cycle(Engine &E, int measure, const Plan &P, std::vector<float> &launch_ms, float &phase_ms)
+{
+    int rc;
+    if (P.concurrent && (rc = ensure_side(E, P.launches.size()))) return rc;
+    Probe probe;
+    hipEvent_t start = nullptr;
+    HIPCHK(hipEventCreate(&start));
+    HIPCHK(hipEventCreate(&probe.phase_end));
+    if (!P.concurrent)
+        for (size_t i = 0; i < P.launches.size(); ++i) {
+            std::pair<hipEvent_t, hipEvent_t> ev;
+            HIPCHK(hipEventCreate(&ev.first));
+            HIPCHK(hipEventCreate(&ev.second));
+            probe.per.push_back(ev);
+        }
+    rc = 0;
+    if (hipEventRecord(start, E.stream) != hipSuccess) rc = fail(J1J2_EDEVICE, "hipEventRecord failed");
+    if (!rc) rc = enqueue_cycle(E, measure, &P, &probe);
+    if (!rc && hipStreamSynchronize(E.stream) != hipSuccess) rc = fail(J1J2_EDEVICE, "hipStreamSynchronize failed");
+    launch_ms.assign(probe.per.size(), 0.0f);
+    phase_ms = 0.0f;
+    if (!rc) {
+        for (size_t i = 0; i < probe.per.size(); ++i)
+            (void)hipEventElapsedTime(&launch_ms[i], probe.per[i].first, probe.per[i].second);
+        (void)hipEventElapsedTime(&phase_ms, start, probe.phase_end);
+    }
+    for (auto &ev : probe.per) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    (void)hipEventDestroy(probe.phase_end);
+    (void)hipEventDestroy(start);
+    return rc;
+}
+
+std::string tune_key(const Engine &E)
+{
+    hipDeviceProp_t prop{};
+    (void)hipGetDeviceProperties(&prop, E.device);
+    std::string k = std::string(prop.gcnArchName) + (E.shadow ? " shadow" : "");
+    for (const Group &G : E.groups)
+        k += " " + std::to_string(G.p.L) + "x" + std::to_string(G.p.H) + ":" + std::to_string(G.nlat) +
+             (G.resident ? "r" : "g");
+    return k;
+}
+
+// Runs up to TUNE_CYCLES cycles of the call (counted in `done`) when they fit.
+constexpr int TUNE_REPS = 2;
+
+int tune(Engine &E, int measure, int ncycles, int &done)
+{
+    std::vector<int> R;
+    for (int g = 0; g < (int)E.groups.size(); ++g)
+        if (E.groups[g].resident) R.push_back(g);
+    if (R.empty()) {
+        E.tuned = true;
+        return 0;
+    }
+    if (E.tune_key.empty()) E.tune_key = tune_key(E);
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        auto it = tune_cache().find(E.tune_key);
+        if (it != tune_cache().end()) {
+            E.plan = it->second;
+            E.tuned = true;
+            drop_graphs(E);
+            return ensure_side(E, E.plan.concurrent ? E.plan.launches.size() : 0);
+        }
+    }
+    std::vector<std::vector<int32_t>> cand(R.size());
+    size_t maxc = 0;
+    for (size_t i = 0; i < R.size(); ++i) {
+        for (int32_t nt : {256, 512, 576, 1024})
+            if (resident_nt_valid(E.groups[R[i]].p.L, nt)) cand[i].push_back(nt);
+        maxc = std::max(maxc, cand[i].size());
+    }
+    const bool multi = R.size() > 1;
+    const int need = TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_resident ? 3 : 0)) : 0);
+    if (ncycles - done < need) return 0;             // too short a call; tune on a later one
+    int rc;
+    // 1. threads per block of each group, its launch timed alone
+    std::vector<std::vector<float>> best(R.size());
+    for (size_t i = 0; i < R.size(); ++i) best[i].assign(cand[i].size(), 1e30f);
+    for (int rep = 0; rep < TUNE_REPS; ++rep)
+        for (size_t r = 0; r < maxc; ++r) {
+            Plan P;
+            std::vector<size_t> pick(R.size());
+            for (size_t i = 0; i < R.size(); ++i) {
+                pick[i] = std::min(r, cand[i].size() - 1);
+                P.launches.push_back(group_launch(E, R[i], cand[i][pick[i]]));
+            }
+            std::vector<float> ms;
+            float phase = 0.0f;
+            if ((rc = tune_cycle(E, measure, P, ms, phase))) return rc;
+            ++done;
+            for (size_t i = 0; i < R.size(); ++i) best[i][pick[i]] = std::min(best[i][pick[i]], ms[i]);
+        }
+    Plan per;
+    float t_seq = 0.0f;
+    for (size_t i = 0; i < R.size(); ++i) {
+        const size_t b = (size_t)(std::min_element(best[i].begin(), best[i].end()) - best[i].begin());
+        per.launches.push_back(group_launch(E, R[i], cand[i][b]));
+        t_seq += best[i][b];
+    }
+    Plan choice = per;
+    float t_best = t_seq;
+    if (multi) {
+        // 2. the same launches on parallel branches, and one launch over all
+        std::vector<Plan> alt;
+        Plan conc = per;
+        conc.concurrent = true;
+        alt.push_back(conc);
+        if (E.all_resident)
+            for (int32_t nt : {256, 512, 1024}) {
+                Plan one;
+                one.launches.push_back(all_launch(E, nt));
+                alt.push_back(one);
+            }
+        for (const Plan &P : alt) {
+            float tmin = 1e30f;
+            for (int rep = 0; rep < TUNE_REPS; ++rep) {
+                std::vector<float> ms;
+                float phase = 0.0f;
+                if ((rc = tune_cycle(E, measure, P, ms, phase))) return rc;
+                ++done;
+                tmin = std::min(tmin, phase);
+            }
+            if (tmin < t_best) {
+                t_best = tmin;
+                choice = P;
+            }
+        }
+    }
+    E.plan = choice;
+    E.tuned = true;
+    drop_graphs(E);
+    {
+        std::lock_guard<std::mutex> lk(tune_mutex());
+        tune_cache()[E.tune_key] = choice;
+    }
+    return ensure_side(E, choice.concurrent ? choice.launches.size() : 0);
+}
+
 int run_cycles(Engine &E, int phase, int ncycles, int begin)
 {
     if (phase != J1J2_PHASE_EQUIL && phase != J1J2_PHASE_MEAS) return fail(J1J2_EINVAL, "unknown phase");
@@ -417,9 +718,10 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
             for (int c = 0; c < (int)E.confs.size(); ++c)
                 if ((rc = reset_measurement(E, c))) return rc;
     }
-    const bool graph = E.use_graph && !E.timing;
     int done = 0;
-    if (graph && ncycles >= GRAPH_CYCLES) {
+    if (!E.tuned && E.autotune && !E.timing && (rc = tune(E, measure, ncycles, done))) return rc;
+    const bool graph = E.use_graph && !E.timing;
+    if (graph && ncycles - done >= GRAPH_CYCLES) {
         if ((rc = build_graph(E, measure, 1))) return rc;
         for (; done + GRAPH_CYCLES <= ncycles; done += GRAPH_CYCLES)
             HIPCHK(hipGraphLaunch(E.graph[measure][1], E.stream));
@@ -566,7 +868,13 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     E->uniform_L = c0.L;
     for (const Group &G : E->groups)
         if (G.p.L != c0.L) E->uniform_L = 0;
-    if (const char *nt = std::getenv("J1J2_RES_THREADS")) E->res_nt = std::atoi(nt) == 512 ? 512 : 256;
+    if (const char *nt = std::getenv("J1J2_RES_THREADS")) {
+        // a fixed block size for every resident launch, no autotuning
+        const int v = std::atoi(nt);
+        E->res_nt = (v == 512 || v == 1024) ? v : 256;
+        E->autotune = false;
+    }
+    if (const char *at = std::getenv("J1J2_AUTOTUNE")) E->autotune = E->autotune && std::atoi(at) != 0;
     if (E->all_resident || std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.resident; }))
         HIPCHK(setup_resident((size_t)E->max_n));
 
@@ -602,6 +910,27 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;
     st.seed = (uint32_t)c0.seed;
     st.fuse_locnem = (E->locnem && E->all_resident) ? 1 : 0;
+    {
+        // work lists of the resident launches: per resident group, and over all lattices
+        std::vector<int32_t> order;
+        E->group_seg.assign(E->groups.size(), OrderSeg{});
+        auto add = [&](int c0, int c1) {
+            const std::vector<int32_t> seg = make_order(*E, c0, c1);
+            OrderSeg o;
+            o.ord0 = (int32_t)order.size();
+            o.nord = (int32_t)seg.size();
+            order.insert(order.end(), seg.begin(), seg.end());
+            return o;
+        };
+        for (int g = 0; g < (int)E->groups.size(); ++g)
+            if (E->groups[g].resident) E->group_seg[g] = add(E->groups[g].conf0, E->groups[g].conf0 + E->groups[g].p.nconf);
+        if (E->all_resident) E->all_seg = add(0, (int)nconf);
+        int32_t *dorder;
+        if ((rc = E->alloc(dorder, (int64_t)order.size()))) return rc;
+        if (!order.empty() && (rc = h2d(*E, dorder, order))) return rc;
+        st.order = dorder;
+    }
+    E->plan = default_plan(*E);
     if ((rc = h2d(*E, lats, E->lats)) || (rc = h2d(*E, confs, E->confs)) || (rc = h2d(*E, cconst, E->cconst)))
         return rc;
     std::vector<double> zero_p((size_t)E->max_n, 0.0);
@@ -1002,6 +1331,18 @@ int j1j2_sweep_path(void *handle, int32_t *path, double *bytes)
     // algorithmic HBM bytes per site update: spin read + write (1 + 1 B) and the
     // site's disorder value (FP32 4 B or FP64 8 B)
     if (bytes) *bytes = E->path == 0 ? 10.0 : 6.0;
+    return 0;
+}
+
+int j1j2_launch_plan(void *handle, char *buf, int32_t len)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (!buf || len <= 0) return fail(J1J2_EINVAL, "need a buffer");
+    std::string d;
+    if (E->plan.launches.empty()) d = "no resident launches";
+    else d = E->plan.describe() + (E->tuned ? " (autotuned)" : " (default)");
+    std::snprintf(buf, (size_t)len, "%s", d.c_str());
     return 0;
 }
 

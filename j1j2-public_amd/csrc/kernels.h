@@ -23,6 +23,7 @@ struct DeviceState {
     const ConfConst *cconst;   // nconf
     int32_t nlat, nconf;
     int64_t nsites;            // sum over configs of N (per-site planes)
+    const int32_t *order;      // resident launches: lattice of each block (-1: none), per launch segment
     // lattice state
     uint8_t *spins;            // sum over lattices of N bytes, internal layout
     const float *kd;           // 5 planes x nsites: kc, then 2^14 * (2Kx+, 2Kx-, 2Ky+, 2Ky-)
@@ -60,12 +61,16 @@ struct DeviceState {
 };
 
 // LDS-resident FP32 sweep of `ptswapever` sweeps + observables, one block of
-// nt (256 or 512) threads per lattice in lats[lat_begin, lat_begin + nlat).
-// Requires resident_supported(L) for every lattice of the range.
-// L: the common lattice size of the range, or 0 for a range of mixed sizes.
-hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat,
+// nt threads per lattice: block b takes lattice order[ord0 + b] (nord blocks,
+// a multiple of 8; -1 entries are idle).  Requires resident_supported(L) for
+// every lattice and resident_nt_valid(L, nt).  L: the common lattice size of
+// the segment, or 0 for mixed sizes.  The result depends neither on nt nor on
+// the order.
+hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord,
                                  int32_t max_n, int32_t L, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);
+// nt in {256, 512, 1024} for every L (and L = 0, mixed); 576 for L = 24, 48.
+bool resident_nt_valid(int32_t L, int32_t nt);
 size_t resident_lds_limit();
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).
 hipError_t setup_resident(size_t max_lds);

@@ -307,7 +307,7 @@ template <int WPH, int NT, int HX>
 __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const int32_t *qh,
                                              const int32_t *ql, ObsFast &a)
 {
-    constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = NT / PER_Z, STEP = ZSTEP * PER_Z;
+    constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = NT / PER_Z;
     const int nq = PER_Z * H, tid = threadIdx.x;
     const int tz = tid % PER_Z, w = tz % WPH;
     int z0 = tid / PER_Z;
@@ -623,37 +623,36 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; consecutive lattices (the
-// replicas of one config, which share the disorder planes) go to the same XCD
-// so those planes are served from that XCD's L2.
-__device__ __forceinline__ int xcd_lattice(int per_xcd) { return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3); }
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8).
+// The host's work list puts all replicas of a config (which share its
+// disorder planes) on one XCD, so those planes are served from that XCD's L2,
+// balances the XCDs by work and orders each XCD's lattices largest first.
 
-// One geometry.  At most 128 VGPRs (four waves per SIMD) so that four
-// 256-thread blocks, i.e. four L = 32 lattices (32 KiB of LDS each), stay
-// resident on every CU.
+
+// One geometry.  At most 128 VGPRs (four waves per SIMD): e.g. four 256-thread
+// blocks, i.e. four L = 32 lattices (32 KiB of LDS each), or one 1024-thread
+// block stay resident on every CU.
 template <int WPH, int NT, bool SHADOW>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : NT / 64, 8)))
-void k_sweep_resident(DeviceState st, int32_t lat_begin, int32_t nlat, int32_t per_xcd)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
+void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
 {
     // the lattice at LDS address 0 (so neighbour addresses need no base), the
     // reduction scratch right after it
     extern __shared__ uint32_t lds32[];
-    const int li = xcd_lattice(per_xcd);
-    if (li >= nlat) return;
-    const int lat = lat_begin + li;
+    const int lat = st.order[ord0 + blockIdx.x];
+    if (lat < 0) return;
     const LatDesc d = st.lats[lat];
     resident_body<WPH, NT, SHADOW>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4));
 }
 
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
 template <int NT, bool SHADOW>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : NT / 64, 8)))
-void k_sweep_resident_mixed(DeviceState st, int32_t lat_begin, int32_t nlat, int32_t per_xcd)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
+void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
 {
     extern __shared__ uint32_t lds32[];
-    const int li = xcd_lattice(per_xcd);
-    if (li >= nlat) return;
-    const int lat = lat_begin + li;
+    const int lat = st.order[ord0 + blockIdx.x];
+    if (lat < 0) return;
     const LatDesc d = st.lats[lat];
     int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4);
     switch (d.L) {
@@ -975,7 +974,7 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
 __global__ void k_begin_phase(DeviceClock *clk) { clk->phase_start = clk->cycle; }
 
 // reduction scratch after the lattice in the resident sweep's dynamic LDS
-constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (512 / 64) * OBS_N;
+constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (1024 / 64) * OBS_N;
 
 }  // namespace
 
@@ -988,7 +987,7 @@ size_t resident_lds_limit()
     return lim > RES_STATIC_LDS ? lim - RES_STATIC_LDS : 0;
 }
 
-using ResKernel = void (*)(DeviceState, int32_t, int32_t, int32_t);
+using ResKernel = void (*)(DeviceState, int32_t, int32_t);
 
 template <int NT, bool SHADOW>
 ResKernel resident_kernel(int32_t L)
@@ -1004,34 +1003,54 @@ ResKernel resident_kernel(int32_t L)
     }
 }
 
+// 576 threads = 4 z-layers of an L = 24 or L = 48 colour plane per step (the
+// fixed-geometry path); only those two sizes are built with it.
+template <bool SHADOW>
+ResKernel resident_kernel_576(int32_t L)
+{
+    switch (L) {
+        case 24: return k_sweep_resident<3, 576, SHADOW>;
+        case 48: return k_sweep_resident<6, 576, SHADOW>;
+        default: return nullptr;
+    }
+}
+
 ResKernel pick_resident(int32_t L, int32_t nt, bool shadow)
 {
-    if (nt == 512) return shadow ? resident_kernel<512, true>(L) : resident_kernel<512, false>(L);
-    return shadow ? resident_kernel<256, true>(L) : resident_kernel<256, false>(L);
+    switch (nt) {
+        case 256: return shadow ? resident_kernel<256, true>(L) : resident_kernel<256, false>(L);
+        case 512: return shadow ? resident_kernel<512, true>(L) : resident_kernel<512, false>(L);
+        case 576: return shadow ? resident_kernel_576<true>(L) : resident_kernel_576<false>(L);
+        case 1024: return shadow ? resident_kernel<1024, true>(L) : resident_kernel<1024, false>(L);
+        default: return nullptr;
+    }
 }
 
 hipError_t setup_resident(size_t max_lds)
 {
     for (int L : {8, 16, 24, 32, 40, 48, 0})
-        for (int nt : {256, 512})
+        for (int nt : {256, 512, 576, 1024})
             for (bool sh : {false, true}) {
-                hipError_t e = hipFuncSetAttribute((const void *)pick_resident(L, nt, sh),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                const ResKernel k = pick_resident(L, nt, sh);
+                if (!k) continue;
+                hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                    (int)(max_lds + RES_STATIC_LDS));
                 if (e != hipSuccess) return e;
             }
     return hipSuccess;
 }
 
+bool resident_nt_valid(int32_t L, int32_t nt) { return pick_resident(L, nt, false) != nullptr; }
+
 bool resident_supported(int32_t L) { return L >= 8 && L <= 48 && (L % 8) == 0; }
 
-hipError_t launch_sweep_resident(const DeviceState &st, int32_t lat_begin, int32_t nlat, int32_t max_n,
+hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord, int32_t max_n,
                                  int32_t L, int32_t nt, bool shadow, hipStream_t s)
 {
-    if (nlat <= 0) return hipSuccess;
-    const int per_xcd = (nlat + 7) / 8;
-    hipLaunchKernelGGL(pick_resident(L, nt, shadow), dim3(8 * per_xcd), dim3(nt == 512 ? 512 : 256),
-                       (size_t)max_n + RES_STATIC_LDS, s, st, lat_begin, nlat, per_xcd);
+    if (nord <= 0) return hipSuccess;
+    const ResKernel k = pick_resident(L, nt, shadow);
+    if (!k) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(nord), dim3(nt), (size_t)max_n + RES_STATIC_LDS, s, st, ord0, nord);
     return hipGetLastError();
 }
 

@@ -220,6 +220,13 @@ module j1j2_engine_mod
          real(c_double), intent(out) :: bytes_per_update
       end function
 
+      integer(c_int) function j1j2_launch_plan(handle, buf, len) bind(C, name='j1j2_launch_plan')
+         import :: c_int, c_int32_t, c_char, c_ptr
+         type(c_ptr), value :: handle
+         character(kind=c_char), intent(out) :: buf(*)
+         integer(c_int32_t), value :: len
+      end function
+
       integer(c_int) function j1j2_synchronize(handle) bind(C, name='j1j2_synchronize')
          import :: c_int, c_ptr
          type(c_ptr), value :: handle

@@ -27,7 +27,7 @@ EXPORTS = [
     "j1j2_upload_disorder", "j1j2_set_spins", "j1j2_set_temperatures", "j1j2_run",
     "j1j2_run_cycles", "j1j2_fetch", "j1j2_get_spins", "j1j2_get_maps", "j1j2_get_observables",
     "j1j2_get_sums", "j1j2_get_series", "j1j2_get_flips", "j1j2_get_mismatch", "j1j2_set_timing",
-    "j1j2_get_timing", "j1j2_sweep_path", "j1j2_synchronize",
+    "j1j2_get_timing", "j1j2_sweep_path", "j1j2_launch_plan", "j1j2_synchronize",
 ]
 
 
@@ -92,6 +92,7 @@ def _declare(L):
         "j1j2_set_timing": [P, i32],
         "j1j2_get_timing": [P, i32, P, P],
         "j1j2_sweep_path": [P, P, P],
+        "j1j2_launch_plan": [P, ctypes.c_char_p, i32],
         "j1j2_synchronize": [P],
     }
     for name, args in sig.items():
@@ -268,6 +269,11 @@ class Engine:
         p, b = np.zeros(1, np.int32), np.zeros(1)
         check(lib().j1j2_sweep_path(self._h, _p(p), _p(b)))
         return int(p[0]), float(b[0])
+
+    def launch_plan(self):
+        buf = ctypes.create_string_buffer(512)
+        check(lib().j1j2_launch_plan(self._h, buf, 512))
+        return buf.value.decode()
 
     def synchronize(self):
         check(lib().j1j2_synchronize(self._h))
