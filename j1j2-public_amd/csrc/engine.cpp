@@ -605,6 +605,7 @@ std::string tune_key(const Engine &E)
 
 // Runs up to TUNE_CYCLES cycles of the call (counted in `done`) when they fit.
 constexpr int TUNE_REPS = 2;
+constexpr float TUNE_TIE = 1.02f;
 
 int tune(Engine &E, int measure, int ncycles, int &done)
 {
@@ -656,8 +657,12 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         }
     Plan per;
     float t_seq = 0.0f;
+    // a candidate within TUNE_TIE of the fastest counts as a tie and the
+    // earlier one (fewer threads) is kept, so that the choice is stable
     for (size_t i = 0; i < R.size(); ++i) {
-        const size_t b = (size_t)(std::min_element(best[i].begin(), best[i].end()) - best[i].begin());
+        const float tmin = *std::min_element(best[i].begin(), best[i].end());
+        size_t b = 0;
+        while (best[i][b] > tmin * TUNE_TIE) ++b;
         per.launches.push_back(group_launch(E, R[i], cand[i][b]));
         t_seq += best[i][b];
     }
@@ -684,7 +689,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
                 ++done;
                 tmin = std::min(tmin, phase);
             }
-            if (tmin < t_best) {
+            if (tmin * TUNE_TIE < t_best) {
                 t_best = tmin;
                 choice = P;
             }
