@@ -100,6 +100,13 @@ int j1j2_ladder(const j1j2_params *p, double *temps);
 int j1j2_make_disorder(int32_t L, int32_t H, double xi, int32_t seed, int32_t map_to_uniform,
                        double phi, double *out);
 
+/* The same field for configs iconf = 1 .. nconf of a run at once (seeds
+ * seed0 + k, k = iconf - 1; seed0 = IRINIT as in Setup_phi_table,
+ * J1J2_6tv.f90:611), on nthreads host threads (0: all cores).  out holds
+ * nconf * L*L*H values, config-major; each field equals j1j2_make_disorder's. */
+int j1j2_make_disorder_batch(int32_t L, int32_t H, double xi, int32_t seed0, int32_t nconf, int32_t map_to_uniform,
+                             double phi, int32_t nthreads, double *out);
+
 /* Initialize_spin_array (J1J2_6tv.f90:1179-1209) for one replica: start 0 = HOT
  * (KISS05 seeded with `seed`, then `skip` draws discarded, then one draw per
  * site: IRINIT+(iconf-1)*NREPLICA+rep and skip 0 for Initialize_pt_state,

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -14,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_model.h"
@@ -978,6 +980,25 @@ int j1j2_make_disorder(int32_t L, int32_t H, double xi, int32_t seed, int32_t ma
 {
     if (!out || L < 1 || H < 1) return fail(J1J2_EINVAL, "invalid disorder request");
     return corr_field(L, L, H, xi, seed, map_to_uniform != 0, phi, out);
+}
+
+int j1j2_make_disorder_batch(int32_t L, int32_t H, double xi, int32_t seed0, int32_t nconf, int32_t map_to_uniform,
+                             double phi, int32_t nthreads, double *out)
+{
+    if (!out || L < 1 || H < 1 || nconf < 0) return fail(J1J2_EINVAL, "invalid disorder request");
+    const int64_t N = (int64_t)L * L * H;
+    int nt = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
+    nt = std::max(1, std::min(nt, nconf));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k; (k = next.fetch_add(1)) < nconf;)
+            (void)corr_field(L, L, H, xi, seed0 + k, map_to_uniform != 0, phi, out + (int64_t)k * N);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    return 0;
 }
 
 int j1j2_initial_spins(int32_t L, int32_t H, int32_t start, int32_t straxis, int32_t seed, int64_t skip,

@@ -50,6 +50,14 @@ module j1j2_engine_mod
          real(c_double), intent(out) :: out(*)
       end function
 
+      integer(c_int) function j1j2_make_disorder_batch(L, H, xi, seed0, nconf, map_to_uniform, phi, nthreads, &
+                                                       out) bind(C, name='j1j2_make_disorder_batch')
+         import :: c_int, c_int32_t, c_double
+         integer(c_int32_t), value :: L, H, seed0, nconf, map_to_uniform, nthreads
+         real(c_double), value :: xi, phi
+         real(c_double), intent(out) :: out(*)
+      end function
+
       integer(c_int) function j1j2_initial_spins(L, H, start, straxis, seed, skip, spins) &
          bind(C, name='j1j2_initial_spins')
          import :: c_int, c_int32_t, c_int64_t, c_int8_t

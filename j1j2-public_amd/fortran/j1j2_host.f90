@@ -248,16 +248,22 @@ contains
       end if
    end function
 
-   ! Setup_phi_table (J1J2_6tv.f90:590-633) for config iconf (engine index k).
-   subroutine upload_field(eng, k, iconf, field)
-      type(c_ptr), intent(in) :: eng
-      integer, intent(in) :: k, iconf
-      real(r8b), intent(out) :: field(0:L3 - 1)
+   ! Setup_phi_table (J1J2_6tv.f90:590-633) for the configs first .. first+n-1
+   ! at once (seeds IRINIT + iconf - 1, on all host cores).
+   subroutine make_fields(first, n, fields)
+      integer, intent(in) :: first, n
+      real(r8b), intent(out) :: fields(0:L3 - 1, n)
       integer(i4b) :: map
       map = 0
       if (MAPTOUNIFORM) map = 1
-      call j1j2_check(j1j2_make_disorder(L, H, corr_length, IRINIT + iconf - 1, map, phi, field), &
-                      'correlated field')
+      call j1j2_check(j1j2_make_disorder_batch(L, H, corr_length, IRINIT + first - 1, int(n, i4b), map, phi, &
+                                               0_i4b, fields), 'correlated field')
+   end subroutine
+
+   subroutine upload_field(eng, k, iconf, field)
+      type(c_ptr), intent(in) :: eng
+      integer, intent(in) :: k, iconf
+      real(r8b), intent(in) :: field(0:L3 - 1)
       call j1j2_check(j1j2_upload_disorder(eng, int(k, i4b), field, int(L3, c_int64_t)), 'upload field')
       if (WRITEPHITABLE) jointp(:, iconf) = field
    end subroutine
@@ -291,7 +297,7 @@ contains
       integer, intent(in) :: first, n
       type(j1j2_params) :: p
       type(c_ptr) :: eng
-      real(r8b), allocatable :: field(:), bf(:, :), hist(:, :), hmin(:), hbin(:), locspin(:), locnem(:)
+      real(r8b), allocatable :: fields(:, :), bf(:, :), hist(:, :), hmin(:), hbin(:), locspin(:), locnem(:)
       integer(i1b), allocatable :: spins(:, :)
       integer(i4b), allocatable :: att(:), acc(:), flips(:, :)
       integer(i4b) :: visit, nsamp
@@ -302,14 +308,15 @@ contains
       p%nconf = n
       p%iconf0 = first
       call j1j2_check(j1j2_create(p, DEVICE, eng), 'engine')
-      allocate (field(0:L3 - 1), spins(0:L3 - 1, NTEMP), bf(NTEMP, J1J2_BASEFIELDS), hist(NTEMP, BINS))
+      allocate (fields(0:L3 - 1, n), spins(0:L3 - 1, NTEMP), bf(NTEMP, J1J2_BASEFIELDS), hist(NTEMP, BINS))
       allocate (hmin(NTEMP), hbin(NTEMP), locspin(0:L3 - 1), locnem(0:L3 - 1))
       allocate (att(max(1, NTEMP - 1)), acc(max(1, NTEMP - 1)), flips(0:L3 - 1, NTEMP))
 
+      call make_fields(first, n, fields)
       do k = 0, n - 1
          iconf = first + k
          print *, 'dis. conf.', iconf
-         call upload_field(eng, k, iconf, field)
+         call upload_field(eng, k, iconf, fields(:, k + 1))
          ! Initialize_pt_state: replica rep starts from KISS seeded with
          ! IRINIT + (iconf-1)*NREPLICA + rep.
          do rep = 1, NTEMP
@@ -318,7 +325,7 @@ contains
          end do
          call j1j2_check(j1j2_set_spins(eng, int(k, i4b), spins, NTEMP), 'set spins')
       end do
-      call write_phi_sites(field)
+      call write_phi_sites(fields(:, n))
 
       call system_clock(t0, rate)
       call j1j2_check(j1j2_run(eng, J1J2_PHASE_EQUIL, -1_i4b), 'equilibration')
@@ -360,7 +367,7 @@ contains
       integer, intent(in) :: first, n
       type(j1j2_params) :: p
       type(c_ptr) :: eng
-      real(r8b), allocatable :: field(:), bf(:, :), hist(:, :), locspin(:), locnem(:)
+      real(r8b), allocatable :: fields(:, :), bf(:, :), hist(:, :), locspin(:), locnem(:)
       integer(i1b), allocatable :: spins(:)
       integer(i4b), allocatable :: flips(:), prev(:, :)
       real(r8b) :: hmin(1), hbin(1), t(1)
@@ -377,7 +384,7 @@ contains
       p%tmin = temperature_for_index(1)
       p%tmax = p%tmin
       call j1j2_check(j1j2_create(p, DEVICE, eng), 'engine')
-      allocate (field(0:L3 - 1), spins(0:L3 - 1), bf(1, J1J2_BASEFIELDS), hist(1, BINS))
+      allocate (fields(0:L3 - 1, n), spins(0:L3 - 1), bf(1, J1J2_BASEFIELDS), hist(1, BINS))
       allocate (locspin(0:L3 - 1), locnem(0:L3 - 1), flips(0:L3 - 1), prev(0:L3 - 1, 0:n - 1))
       prev = 0
 
@@ -386,17 +393,18 @@ contains
       ! horizontal and vertical bond masks, one draw per site each.
       skip = 2_c_int64_t*L3
       if (IMPMODE == 'RAN') skip = 3_c_int64_t*L3
+      call make_fields(first, n, fields)
       do k = 0, n - 1
          iconf = first + k
          print *, 'dis. conf.', iconf
-         call upload_field(eng, k, iconf, field)
+         call upload_field(eng, k, iconf, fields(:, k + 1))
          if (TEMPPROTOCOL == 'CARRY') then
             call j1j2_check(j1j2_initial_spins(L, H, start_code(), axis_code(), IRINIT + iconf - 1, skip, &
                                                spins), 'initial spins')
             call j1j2_check(j1j2_set_spins(eng, int(k, i4b), spins, 1_i4b), 'set spins')
          end if
       end do
-      call write_phi_sites(field)
+      call write_phi_sites(fields(:, n))
 
       do itemp = 1, NTEMP
          t(1) = temperature_for_index(itemp)

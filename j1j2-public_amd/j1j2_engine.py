@@ -22,7 +22,7 @@ PATH_NAMES = {0: "fp64_plain", 1: "fp32_resident", 2: "fp32_global"}
 
 # every function declared in include/j1j2_engine.h
 EXPORTS = [
-    "j1j2_ladder", "j1j2_make_disorder", "j1j2_initial_spins", "j1j2_create", "j1j2_create_batch",
+    "j1j2_ladder", "j1j2_make_disorder", "j1j2_make_disorder_batch", "j1j2_initial_spins", "j1j2_create", "j1j2_create_batch",
     "j1j2_destroy", "j1j2_last_error", "j1j2_num_configs", "j1j2_config_shape",
     "j1j2_upload_disorder", "j1j2_set_spins", "j1j2_set_temperatures", "j1j2_run",
     "j1j2_run_cycles", "j1j2_fetch", "j1j2_get_spins", "j1j2_get_maps", "j1j2_get_observables",
@@ -70,6 +70,7 @@ def _declare(L):
     sig = {
         "j1j2_ladder": [ctypes.POINTER(Params), P],
         "j1j2_make_disorder": [i32, i32, d, i32, i32, d, P],
+        "j1j2_make_disorder_batch": [i32, i32, d, i32, i32, i32, d, i32, P],
         "j1j2_initial_spins": [i32, i32, i32, i32, i32, i64, P],
         "j1j2_create": [ctypes.POINTER(Params), i32, ctypes.POINTER(P)],
         "j1j2_create_batch": [P, i32, i32, ctypes.POINTER(P)],
@@ -134,6 +135,13 @@ def ladder(params):
 def make_disorder(L, H, xi=3.0, seed=1, map_to_uniform=True, phi=2.0):
     out = np.zeros(L * L * H, np.float64)
     check(lib().j1j2_make_disorder(L, H, xi, seed, 1 if map_to_uniform else 0, phi, _p(out)))
+    return out
+
+
+def make_disorder_batch(L, H, xi=3.0, seed0=1, nconf=1, map_to_uniform=True, phi=2.0, nthreads=0):
+    out = np.zeros((nconf, L * L * H), np.float64)
+    check(lib().j1j2_make_disorder_batch(L, H, xi, seed0, nconf, 1 if map_to_uniform else 0, phi, nthreads,
+                                         _p(out)))
     return out
 
 

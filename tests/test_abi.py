@@ -81,3 +81,13 @@ def test_create_without_gpu_reports_device_error():
     rc = J.lib().j1j2_create(ctypes.byref(J.make_params(8, nreplica=4)), 0, ctypes.byref(h))
     assert rc in (-1, -3) and not h.value
     assert J.last_error()
+
+
+def test_disorder_batch_equals_per_config_fields():
+    """j1j2_make_disorder_batch: config k's field is j1j2_make_disorder's for
+    seed seed0 + k, whatever the thread count (host-only, no device)."""
+    for L, H, seed0, n in ((8, 8, 1, 5), (6, 4, 7, 3), (12, 10, 11, 2)):
+        ref = np.stack([J.make_disorder(L, H, 3.0, seed0 + k, True, 2.0) for k in range(n)])
+        for nt in (1, 3, 0):
+            got = J.make_disorder_batch(L, H, 3.0, seed0, n, True, 2.0, nthreads=nt)
+            assert np.array_equal(got, ref)
