@@ -46,7 +46,8 @@ constexpr double LOG2E = 0x1.71547652b82fep+0;
 struct Group {
     j1j2_params p;
     int32_t conf0, lat0, nlat, N;
-    bool resident;
+    bool resident;             // swept by an LDS-resident kernel (observables fused)
+    int kind = 0;              // 0 global-memory passes, 1 resident bytes, 2 resident bits
 };
 
 struct Timer {
@@ -60,6 +61,7 @@ struct Timer {
 // per block, nlat lattices.
 struct ResLaunch {
     int32_t ord0, nord, L, max_n, nt, nlat;
+    int32_t kind = 1, H = 0;   // kind 2: bit layout (nt fixed by L)
 };
 
 // Work-list segment of a lattice range (one group, or all lattices).
@@ -94,8 +96,9 @@ struct Engine {
     std::vector<double> hbound;               // per config: bound on |h| of the FP32 rule
     int64_t nsites = 0, nslots = 0, npairs = 0, nhist = 0, nseries = 0, nflips = 0, spin_bytes = 0;
     int32_t max_n = 0;
-    int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global
+    int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global, 3 resident incl. bits
     bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
+    bool all_byte = false;                     // every group on the byte-layout resident kernel
     int32_t res_nt = 256;                      // threads per block of the resident sweep (default plan)
     int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
     Plan plan;                                 // resident launches of a cycle
@@ -381,9 +384,9 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
                 for (size_t i = 0; i < pl.launches.size() && r == hipSuccess; ++i) {
                     const ResLaunch &L = pl.launches[i];
                     if ((r = hipStreamWaitEvent(e.side[i], e.fork_ev, 0)) != hipSuccess) break;
-                    if ((r = launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.nt, e.shadow, e.side[i])) !=
-                        hipSuccess)
-                        break;
+                    r = L.kind == 2 ? launch_sweep_bits(e.st, L.ord0, L.nord, L.L, L.H, e.shadow, e.side[i])
+                                    : launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.nt, e.shadow, e.side[i]);
+                    if (r != hipSuccess) break;
                     if ((r = hipEventRecord(e.join_ev[i], e.side[i])) != hipSuccess) break;
                     r = hipStreamWaitEvent(e.stream, e.join_ev[i], 0);
                 }
@@ -394,7 +397,9 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
             for (const ResLaunch &L : P.launches) {
                 auto fn = [](Engine &e, const void *a) {
                     const ResLaunch &r = *static_cast<const ResLaunch *>(a);
-                    return launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.nt, e.shadow, e.stream);
+                    return r.kind == 2 ? launch_sweep_bits(e.st, r.ord0, r.nord, r.L, r.H, e.shadow, e.stream)
+                                       : launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.nt, e.shadow,
+                                                               e.stream);
                 };
                 const size_t i = (size_t)(&L - P.launches.data());
                 if (probe && i < probe->per.size()) HIPCHK(hipEventRecord(probe->per[i].first, E.stream));
@@ -487,7 +492,10 @@ std::map<std::string, Plan> &tune_cache()
 ResLaunch group_launch(const Engine &E, int g, int32_t nt)
 {
     const Group &G = E.groups[g];
-    return {E.group_seg[g].ord0, E.group_seg[g].nord, G.p.L, G.N, nt, G.nlat};
+    ResLaunch r{E.group_seg[g].ord0, E.group_seg[g].nord, G.p.L, G.N, nt, G.nlat};
+    r.kind = G.kind;
+    r.H = G.p.H;
+    return r;
 }
 
 ResLaunch all_launch(const Engine &E, int32_t nt)
@@ -498,7 +506,7 @@ ResLaunch all_launch(const Engine &E, int32_t nt)
 Plan default_plan(const Engine &E)
 {
     Plan P;
-    if (E.all_resident) {
+    if (E.all_byte) {
         P.launches.push_back(all_launch(E, E.res_nt));
         return P;
     }
@@ -632,12 +640,14 @@ int tune(Engine &E, int measure, int ncycles, int &done)
     std::vector<std::vector<int32_t>> cand(R.size());
     size_t maxc = 0;
     for (size_t i = 0; i < R.size(); ++i) {
-        for (int32_t nt : {256, 512, 576, 1024})
-            if (resident_nt_valid(E.groups[R[i]].p.L, nt)) cand[i].push_back(nt);
+        if (E.groups[R[i]].kind == 2) cand[i].push_back(0);     // bit layout: block size fixed by L
+        else
+            for (int32_t nt : {256, 512, 576, 1024})
+                if (resident_nt_valid(E.groups[R[i]].p.L, nt)) cand[i].push_back(nt);
         maxc = std::max(maxc, cand[i].size());
     }
     const bool multi = R.size() > 1;
-    const int need = TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_resident ? 3 : 0)) : 0);
+    const int need = TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_byte ? 3 : 0)) : 0);
     if (ncycles - done < need) return 0;             // too short a call; tune on a later one
     int rc;
     // 1. threads per block of each group, its launch timed alone
@@ -676,7 +686,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         Plan conc = per;
         conc.concurrent = true;
         alt.push_back(conc);
-        if (E.all_resident)
+        if (E.all_byte)
             for (int32_t nt : {256, 512, 1024}) {
                 Plan one;
                 one.launches.push_back(all_launch(E, nt));
@@ -821,6 +831,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     int64_t nlat = 0;
     int64_t series_cap = c0.sampler == J1J2_SAMPLER_LOCAL ? std::max(c0.nmess, 1) : 0;
     E->all_resident = true;
+    E->all_byte = true;
     for (int g = 0; g < ngroups; ++g) {
         const j1j2_params &p = groups[g];
         Group G;
@@ -829,9 +840,13 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         G.conf0 = (int32_t)E->confs.size();
         G.lat0 = (int32_t)nlat;
         G.nlat = p.nconf * p.nreplica;
-        G.resident = p.precision == J1J2_FP32_BATCHED && resident_supported(p.L) && (size_t)G.N <= lds_limit &&
-                     !flips && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
+        const bool may = p.precision == J1J2_FP32_BATCHED && !flips && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
+        if (may && resident_supported(p.L) && (size_t)G.N <= lds_limit) G.kind = 1;
+        else if (may && bits_supported(p.L) && bits_lds_need(p.L, p.H) <= lds_limit + RESIDENT_SCRATCH_LDS)
+            G.kind = 2;
+        G.resident = G.kind != 0;
         E->all_resident = E->all_resident && G.resident;
+        E->all_byte = E->all_byte && G.kind == 1;
         E->max_n = std::max(E->max_n, G.N);
         std::vector<double> t(p.nreplica);
         temperature_ladder(p.ladder, p.nreplica, p.tmin, p.tmax, p.dt, p.tladdermid, p.tladderfrac, t.data());
@@ -871,7 +886,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         }
         E->groups.push_back(G);
     }
-    E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_resident ? 1 : 2);
+    E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_byte ? 1 : (E->all_resident ? 3 : 2));
     E->uniform_L = c0.L;
     for (const Group &G : E->groups)
         if (G.p.L != c0.L) E->uniform_L = 0;
@@ -882,8 +897,12 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         E->autotune = false;
     }
     if (const char *at = std::getenv("J1J2_AUTOTUNE")) E->autotune = E->autotune && std::atoi(at) != 0;
-    if (E->all_resident || std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.resident; }))
+    if (std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.kind == 1; }))
         HIPCHK(setup_resident((size_t)E->max_n));
+    size_t bits_lds = 0;
+    for (const Group &G : E->groups)
+        if (G.kind == 2) bits_lds = std::max(bits_lds, bits_lds_need(G.p.L, G.p.H));
+    if (bits_lds) HIPCHK(setup_bits(bits_lds));
 
     DeviceState &st = E->st;
     const int64_t nconf = (int64_t)E->confs.size();
@@ -931,7 +950,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         };
         for (int g = 0; g < (int)E->groups.size(); ++g)
             if (E->groups[g].resident) E->group_seg[g] = add(E->groups[g].conf0, E->groups[g].conf0 + E->groups[g].p.nconf);
-        if (E->all_resident) E->all_seg = add(0, (int)nconf);
+        if (E->all_byte) E->all_seg = add(0, (int)nconf);
         int32_t *dorder;
         if ((rc = E->alloc(dorder, (int64_t)order.size()))) return rc;
         if (!order.empty() && (rc = h2d(*E, dorder, order))) return rc;

@@ -69,6 +69,15 @@ struct DeviceState {
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord,
                                  int32_t max_n, int32_t L, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);
+// Bytes of LDS the resident kernels add after the lattice (reduction scratch).
+constexpr size_t RESIDENT_SCRATCH_LDS = sizeof(int64_t) * (1024 / 64) * 8;
+// Bit-layout resident sweep (kernels_bits.hip): L % 8 == 0, 48 < L <= 96, one
+// bit per site in LDS, block size fixed by L; same results as the byte layout.
+bool bits_supported(int32_t L);
+size_t bits_lds_need(int32_t L, int32_t H);
+hipError_t setup_bits(size_t max_lds);
+hipError_t launch_sweep_bits(const DeviceState &st, int32_t ord0, int32_t nord, int32_t L, int32_t H, bool shadow,
+                             hipStream_t s);
 // nt in {256, 512, 1024} for every L (and L = 0, mixed); 576 for L = 24, 48.
 bool resident_nt_valid(int32_t L, int32_t nt);
 size_t resident_lds_limit();

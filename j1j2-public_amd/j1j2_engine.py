@@ -18,7 +18,7 @@ PHASE_EQUIL, PHASE_MEAS = 0, 1
 FLAG_WRITELOCNEM, FLAG_WRITEHIST, FLAG_WRITENUMFLIPS = 1, 2, 4
 FLAG_SHADOW_FP64, FLAG_NO_GRAPH, FLAG_FORCE_GLOBAL = 8, 16, 32
 START_HOT, START_FER, START_STR = 0, 1, 2
-PATH_NAMES = {0: "fp64_plain", 1: "fp32_resident", 2: "fp32_global"}
+PATH_NAMES = {0: "fp64_plain", 1: "fp32_resident", 2: "fp32_global", 3: "fp32_resident_bits"}
 
 # every function declared in include/j1j2_engine.h
 EXPORTS = [

@@ -62,11 +62,14 @@ def test_uneven_l16_to_48():
     check_properties(eng, inputs, range(8))
 
 
-def test_large_l96_global_path():
-    say("case L=96 24x8")
-    g = J.make_params(96, 96, nreplica=24, nconf=8, seed=SEED)
+@pytest.mark.parametrize("flags,path", [(0, 3), (J.FLAG_FORCE_GLOBAL, 2)])
+def test_large_l96(flags, path):
+    """L = 96, 24 replicas x 8 configs: the bit-layout resident path and the
+    global-memory path."""
+    say(f"case L=96 24x8 path {path}")
+    g = J.make_params(96, 96, nreplica=24, nconf=8, seed=SEED, flags=flags)
     eng, inputs = build_engine([g], "str")
-    assert eng.sweep_path()[0] == 2
+    assert eng.sweep_path()[0] == path
     eng.run(J.PHASE_EQUIL, 1)
     check_properties(eng, inputs, [0, 7])
     gg, iconf, p, s = inputs[3]

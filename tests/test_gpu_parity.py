@@ -44,6 +44,27 @@ def test_fp32_resident_matches_oracle(L, H, R, start):
     assert eng.sweep_path()[0] == 1 and n == 2
 
 
+@pytest.mark.parametrize("L,H,R,start", [(56, 4, 3, "hot"), (64, 2, 2, "str"), (72, 4, 2, "hot"),
+                                          (80, 2, 3, "hot"), (88, 4, 2, "str"), (96, 4, 3, "hot")])
+def test_fp32_resident_bits_matches_oracle(L, H, R, start):
+    """L > 48: the one-bit-per-site LDS layout (kernels_bits.hip)."""
+    say(f"case fp32 resident bits L={L} H={H} R={R} {start}")
+    g = J.make_params(L, H, nreplica=R, nconf=2, iconf0=5, precision=J.FP32_BATCHED, flags=LOCNEM,
+                      seed=SEED, ptmeasindex=min(2, R))
+    eng, n = run_and_compare([g], start, 3, 4)
+    assert eng.sweep_path()[0] == 3 and n == 2
+
+
+def test_bits_shadow_and_mixed_with_bytes():
+    """Bit and byte layouts in one engine (separate launches), with the FP64
+    shadow decisions counted on both, against the oracle."""
+    say("case bits + bytes + shadow")
+    groups = [J.make_params(16, 4, nreplica=2, nconf=1, iconf0=1, flags=J.FLAG_SHADOW_FP64 | LOCNEM, seed=SEED),
+              J.make_params(64, 4, nreplica=3, nconf=1, iconf0=2, flags=J.FLAG_SHADOW_FP64 | LOCNEM, seed=SEED)]
+    eng, n = run_and_compare(groups, "hot", 2, 3, shadow=True)
+    assert eng.sweep_path()[0] == 3 and n == 2
+
+
 @pytest.mark.parametrize("L,H,R", [(6, 4, 3), (2, 4, 2), (10, 6, 4)])
 def test_fp32_global_matches_oracle(L, H, R):
     say(f"case fp32 global L={L} H={H} R={R}")
