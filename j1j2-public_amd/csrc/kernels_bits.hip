@@ -99,74 +99,88 @@ __device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int 
 __device__ __forceinline__ uint32_t nib_bit(uint32_t n, int k) { return (n >> k) & 1u; }
 __device__ __forceinline__ uint32_t sp_bit(uint32_t w, int k) { return (w >> (8 * k + 2)) & 1u; }
 
-// One colour pass of the bit layout.  Thread (yh, j) owns group j of unit
-// (z, yh) for every z; its flips go to the unit word with an LDS atomic xor
-// (the G threads of a unit share the word; nobody reads plane c in pass c).
+// One 4-site group of a colour pass of the bit layout: thread (yh, j) owns
+// group j of unit (z, yh); its flips go to the unit word with an LDS atomic
+// xor (the G threads of a unit share the word; nobody reads plane c in pass c).
+template <int L, int HX, bool SHADOW>
+__device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
+                                          int z, int yh, int j, unsigned long long &sh_prop,
+                                          unsigned long long &sh_mis)
+{
+    using Gm = BitGeo<L>;
+    constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
+    const int tid = threadIdx.x, m0 = 4 * j;
+    const int NU = HW * H, nq = NU * G;
+    unsigned long long *lds64 = reinterpret_cast<unsigned long long *>(lds32);
+    const int u = z * HW + yh;
+    const BitNb n = bit_nb<HW>(c, z, yh, H);
+    const BitSites s = bit_sites<HW, HX>(lds32, c, NU, u, n, m0);
+    const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
+    const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
+    const uint32_t xpe = f16_even(s.xp), xpo = f16_odd(s.xp), xme = f16_even(s.xm), xmo = f16_odd(s.xm);
+    const uint32_t ype = f16_even(s.yp), ypo = f16_odd(s.yp), yme = f16_even(s.ym), ymo = f16_odd(s.ym);
+    const int t = z * TPL + tid;                              // plane-relative group = Philox counter
+    const int own = c * nq + t;
+    const uint32_t ob = (uint32_t)own * 16u;
+    const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob),
+                 kyp = ld16(cx.kyp4, ob), kym = ld16(cx.kym4, ob);
+    const U4 rn = philox4x32_10_u((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+    const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
+    const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
+    const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
+    const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
+    const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
+    const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+    uint32_t f = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float h = kca[k];
+        h = __builtin_fmaf(f16_site(xpe, xpo, k), kxpa[k], h);
+        h = __builtin_fmaf(f16_site(xme, xmo, k), kxma[k], h);
+        h = __builtin_fmaf(f16_site(ype, ypo, k), kypa[k], h);
+        h = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h);
+        h = __builtin_fmaf(byte_f(D, k), cx.j2q, h);
+        h = __builtin_fmaf(byte_f(Z, k), cx.j3q, h);
+        const bool acc = decide_f32_word(h, s.S, k, cx.b2, ra[k]);
+        if constexpr (SHADOW) {
+            const bool up = nib_bit(s.S, k) != 0;
+            const int m = m0 + k;                             // position in the half-row
+            const int mo_p = HX ? ((m == HW - 1) ? 0 : m + 1) : m;
+            const int mo_m = HX ? m : ((m == 0) ? HW - 1 : m - 1);
+            const double sv[10] = {pm1(sp_bit(s.yp, k)), pm1(sp_bit(s.ym, k)), pm1(sp_bit(s.xp, k)),
+                                   pm1(sp_bit(s.xm, k)), pm1(sp_bit(s.d1p, k)), pm1(sp_bit(s.d1m, k)),
+                                   pm1(sp_bit(s.d2p, k)), pm1(sp_bit(s.d2m, k)), pm1(sp_bit(s.zp, k)),
+                                   pm1(sp_bit(s.zm, k))};
+            // site indices in the colour-plane order of the disorder planes
+            const int64_t q1 = (int64_t)((c ^ 1) * NU + u) * HW, q2 = (int64_t)(c ^ 2) * NU * HW;
+            const double pv[5] = {cx.p64[4 * (int64_t)own + k], cx.p64[q2 + (int64_t)n.uyp * HW + m],
+                                  cx.p64[q2 + (int64_t)n.uym * HW + m], cx.p64[q1 + mo_p], cx.p64[q1 + mo_m]};
+            const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[k]);
+            sh_prop += 1;
+            sh_mis += (a64 != acc);
+        }
+        f |= (acc ? 1u : 0u) << k;
+    }
+    unsigned long long mask = (unsigned long long)f << (m0 + 1);
+    if (j == 0) mask |= (unsigned long long)(f & 1u) << (HW + 1);     // copy of site 0
+    if (j == G - 1) mask |= (unsigned long long)(f >> 3);             // copy of site HW-1
+    atomicXor(&lds64[c * NU + u], mask);
+}
+
+// One colour pass: every thread steps through the z-layers, two per trip (the
+// two groups are independent, which the scheduler interleaves).
 template <int L, int HX, bool SHADOW>
 __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
                                          unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
     using Gm = BitGeo<L>;
-    constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
+    constexpr int G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
-    const int yh = tid / G, j = tid - yh * G, m0 = 4 * j;
-    const int NU = HW * H, nq = NU * G;
-    unsigned long long *lds64 = reinterpret_cast<unsigned long long *>(lds32);
-    for (int z = 0; z < H; ++z) {
-        const int u = z * HW + yh;
-        const BitNb n = bit_nb<HW>(c, z, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, NU, u, n, m0);
-        const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
-        const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
-        const uint32_t xpe = f16_even(s.xp), xpo = f16_odd(s.xp), xme = f16_even(s.xm), xmo = f16_odd(s.xm);
-        const uint32_t ype = f16_even(s.yp), ypo = f16_odd(s.yp), yme = f16_even(s.ym), ymo = f16_odd(s.ym);
-        const int t = z * TPL + tid;                              // plane-relative group = Philox counter
-        const int own = c * nq + t;
-        const uint32_t ob = (uint32_t)own * 16u;
-        const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob),
-                     kyp = ld16(cx.kyp4, ob), kym = ld16(cx.kym4, ob);
-        const U4 rn = philox4x32_10_u((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
-        const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
-        const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
-        const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
-        const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
-        const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
-        const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
-        uint32_t f = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float h = kca[k];
-            h = __builtin_fmaf(f16_site(xpe, xpo, k), kxpa[k], h);
-            h = __builtin_fmaf(f16_site(xme, xmo, k), kxma[k], h);
-            h = __builtin_fmaf(f16_site(ype, ypo, k), kypa[k], h);
-            h = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h);
-            h = __builtin_fmaf(byte_f(D, k), cx.j2q, h);
-            h = __builtin_fmaf(byte_f(Z, k), cx.j3q, h);
-            const bool acc = decide_f32_word(h, s.S, k, cx.b2, ra[k]);
-            if constexpr (SHADOW) {
-                const bool up = nib_bit(s.S, k) != 0;
-                const int m = m0 + k;                             // position in the half-row
-                const int mo_p = HX ? ((m == HW - 1) ? 0 : m + 1) : m;
-                const int mo_m = HX ? m : ((m == 0) ? HW - 1 : m - 1);
-                const double sv[10] = {pm1(sp_bit(s.yp, k)), pm1(sp_bit(s.ym, k)), pm1(sp_bit(s.xp, k)),
-                                       pm1(sp_bit(s.xm, k)), pm1(sp_bit(s.d1p, k)), pm1(sp_bit(s.d1m, k)),
-                                       pm1(sp_bit(s.d2p, k)), pm1(sp_bit(s.d2m, k)), pm1(sp_bit(s.zp, k)),
-                                       pm1(sp_bit(s.zm, k))};
-                // site indices in the colour-plane order of the disorder planes
-                const int64_t q1 = (int64_t)((c ^ 1) * NU + u) * HW, q2 = (int64_t)(c ^ 2) * NU * HW;
-                const double pv[5] = {cx.p64[4 * (int64_t)own + k], cx.p64[q2 + (int64_t)n.uyp * HW + m],
-                                      cx.p64[q2 + (int64_t)n.uym * HW + m], cx.p64[q1 + mo_p], cx.p64[q1 + mo_m]};
-                const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[k]);
-                sh_prop += 1;
-                sh_mis += (a64 != acc);
-            }
-            f |= (acc ? 1u : 0u) << k;
-        }
-        unsigned long long mask = (unsigned long long)f << (m0 + 1);
-        if (j == 0) mask |= (unsigned long long)(f & 1u) << (HW + 1);     // copy of site 0
-        if (j == G - 1) mask |= (unsigned long long)(f >> 3);             // copy of site HW-1
-        atomicXor(&lds64[c * NU + u], mask);
+    const int yh = tid / G, j = tid - yh * G;
+    for (int z = 0; z < H; z += 2) {                 // H is even
+        bit_group<L, HX, SHADOW>(lds32, cx, cc, c, H, z, yh, j, sh_prop, sh_mis);
+        bit_group<L, HX, SHADOW>(lds32, cx, cc, c, H, z + 1, yh, j, sh_prop, sh_mis);
     }
 }
 
@@ -352,8 +366,10 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     }
 }
 
+// At most 168 VGPRs (three waves per SIMD): one lattice of L >= 64 fills most
+// of a CU's LDS, so a CU holds one or two blocks of at most 9 waves.
 template <int L, bool SHADOW>
-__global__ __launch_bounds__(BitGeo<L>::NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
+__global__ __launch_bounds__(BitGeo<L>::NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 3, 8)))
 void k_sweep_bits(DeviceState st, int32_t ord0, int32_t nord, int32_t lds_bytes)
 {
     // the lattice's unit words from LDS address 0, the reduction scratch after them
