@@ -112,7 +112,9 @@ __device__ __forceinline__ U4 philox4x32_10_u(uint32_t c0, uint32_t c1, uint32_t
     uint64_t p0 = (uint64_t)M0 * c0;
     uint64_t p1 = (uint64_t)M1 * c2;
     uint32_t a0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;                  // uniform
-    uint32_t a2 = (uint32_t)(p0 >> 32) ^ (c3 ^ k1);                // vector
+    uint32_t u3 = c3 ^ k1;
+    asm volatile("" : "+s"(u3));                                    // kept as one scalar (no reassociation)
+    uint32_t a2 = (uint32_t)(p0 >> 32) ^ u3;                       // vector
     uint32_t a1 = (uint32_t)p1, a3 = (uint32_t)p0;                 // uniform, vector
     k0 += W0; k1 += W1;
     // round 2: p0 uniform, p1 vector

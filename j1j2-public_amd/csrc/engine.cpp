@@ -687,7 +687,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         conc.concurrent = true;
         alt.push_back(conc);
         if (E.all_byte)
-            for (int32_t nt : {256, 512, 1024}) {
+            for (int32_t nt : {256, 512, 576, 1024}) {
                 Plan one;
                 one.launches.push_back(all_launch(E, nt));
                 alt.push_back(one);
@@ -893,7 +893,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     if (const char *nt = std::getenv("J1J2_RES_THREADS")) {
         // a fixed block size for every resident launch, no autotuning
         const int v = std::atoi(nt);
-        E->res_nt = (v == 512 || v == 1024) ? v : 256;
+        E->res_nt = (v == 512 || v == 576 || v == 1024) ? v : 256;
         E->autotune = false;
     }
     if (const char *at = std::getenv("J1J2_AUTOTUNE")) E->autotune = E->autotune && std::atoi(at) != 0;

@@ -78,7 +78,7 @@ size_t bits_lds_need(int32_t L, int32_t H);
 hipError_t setup_bits(size_t max_lds);
 hipError_t launch_sweep_bits(const DeviceState &st, int32_t ord0, int32_t nord, int32_t L, int32_t H, bool shadow,
                              hipStream_t s);
-// nt in {256, 512, 1024} for every L (and L = 0, mixed); 576 for L = 24, 48.
+// nt in {256, 512, 576, 1024} for every L (and L = 0, mixed).
 bool resident_nt_valid(int32_t L, int32_t nt);
 size_t resident_lds_limit();
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).

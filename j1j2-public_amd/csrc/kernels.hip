@@ -211,19 +211,32 @@ __device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uin
 }
 
 // The same observables when a plane's words map onto the threads with a fixed
-// (y/2, w) per thread and z stepping by an even NT/PER_Z (L = 8, 16, 32 with
-// 256 or 512 threads): neighbour offsets are then constant per thread, the
+// (y/2, w) per thread and z stepping by an even count (fast_zs below: all
+// sizes with 1024 threads, L <= 40 with 256 and 512, L != 40 with 576):
+// neighbour offsets are then constant per thread, the
 // y parity too, so the bond counts accumulate raw popcounts (turned into
 // sums of s_i s_j once at the end) and the field term accumulates
 // q_i e_i with e_i = l_i / 2 + 2 in 0..4 (sum q_i l_i = 2 sum q_i e_i -
 // 4 sum q_i, the last sum being a per-config constant).
 
+// The fixed-geometry stepping: a step covers ZS z-layers of a colour plane
+// (NT / PER_Z rounded down to even, so each thread's y parity stays fixed);
+// threads ZS * PER_Z.. of the block idle through the pass.  Used when ZS >= 2
+// and at least 3/4 of the threads work (0 otherwise: the general path).
+template <int WPH, int NT>
+constexpr int fast_zs()
+{
+    constexpr int PER_Z = 4 * WPH * WPH, ZS = (NT / PER_Z) & ~1;
+    return (ZS >= 2 && 4 * ZS * PER_Z >= 3 * NT) ? ZS : 0;
+}
+
 template <int WPH, int NT, int HX>
 __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const int32_t *qh,
                                              const int32_t *ql, ObsFast &a)
 {
-    constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = NT / PER_Z;
+    constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = fast_zs<WPH, NT>();
     const int nq = PER_Z * H, tid = threadIdx.x;
+    if (tid >= ZSTEP * PER_Z) return;
     const int tz = tid % PER_Z, w = tz % WPH;
     int z0 = tid / PER_Z;
     if constexpr (PER_Z % 64 == 0) z0 = __builtin_amdgcn_readfirstlane(z0);
@@ -322,6 +335,7 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     x_from<HX>(*P.d2a, *P.d2b, Xp2, Xm2);
     const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise 4 * (0..4)
     const uint32_t Z = ZP + ZM;                         // bytewise 4 * (0..2)
+    const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
     const uint32_t xpe = f16_even(Xp0), xpo = f16_odd(Xp0), xme = f16_even(Xm0), xmo = f16_odd(Xm0);
     const uint32_t ype = f16_even(YP), ypo = f16_odd(YP), yme = f16_even(YM), ymo = f16_odd(YM);
     const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
@@ -342,8 +356,8 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
         h = __builtin_fmaf(f16_site(xme, xmo, j), kxma[j], h);
         h = __builtin_fmaf(f16_site(ype, ypo, j), kypa[j], h);
         h = __builtin_fmaf(f16_site(yme, ymo, j), kyma[j], h);
-        h = __builtin_fmaf(byte_f(D, j), cx.j2q, h);          // (4D) (2 J2 / 4)
-        h = __builtin_fmaf(byte_f(Z, j), cx.j3q, h);
+        h = __builtin_fmaf(f16_site(de, dod, j), cx.j2d, h);  // (2^-24 4D) (2^24 2 J2 / 4)
+        h = __builtin_fmaf(f16_site(ze, zod, j), cx.j3d, h);
         const bool acc = decide_f32_word(h, S, 8 * j + 2, cx.b2, ra[j]);
         if constexpr (SHADOW) {
             const bool up = lds_bit(S, j) != 0;
@@ -373,12 +387,13 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
 {
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH;    // dwords per z-layer of a plane
     const int nq = PER_Z * H, tid = threadIdx.x;
-    if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) {
-        // Each thread keeps its (y/2, w) and steps z by NT/PER_Z (even, so y's
+    if constexpr (fast_zs<WPH, NT>() >= 2) {
+        // Each thread keeps its (y/2, w) and steps z by ZSTEP (even, so y's
         // parity is fixed): every neighbour offset is constant except at the
-        // z boundary.  Two steps per trip: the second step's LDS addresses are
+        // z boundary.  Four steps per trip: a later step's LDS addresses are
         // the first's plus a constant, which the reads take as an immediate.
-        constexpr int ZSTEP = NT / PER_Z, STEP = ZSTEP * PER_Z;
+        constexpr int ZSTEP = fast_zs<WPH, NT>(), STEP = ZSTEP * PER_Z, TRIP = 4;
+        if (tid >= STEP) return;
         const int tz = tid % PER_Z, w = tz % WPH;
         int z0 = tid / PER_Z;
         if constexpr (PER_Z % 64 == 0) z0 = __builtin_amdgcn_readfirstlane(z0);   // wave-uniform
@@ -395,9 +410,9 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
         // re-added to every address on every trip)
         opaque(P.own); opaque(P.yp); opaque(P.ym); opaque(P.zp); opaque(P.zm); opaque(P.xa); opaque(P.xb);
         opaque(P.d1a); opaque(P.d1b); opaque(P.d2a); opaque(P.d2b);
-        for (int k = 0; k < nk; k += 2) {
+        for (int k = 0; k < nk; k += TRIP) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < TRIP; ++i) {
                 const int kk = k + i;
                 if (kk < nk) {
                     const int z = z0 + kk * ZSTEP;
@@ -408,7 +423,7 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
                                                  sh_mis);
                 }
             }
-            P = nb_shift(P, 2 * STEP);
+            P = nb_shift(P, TRIP * STEP);
         }
     } else {
         for (int t = tid; t < nq; t += NT) {
@@ -446,8 +461,8 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.kym4 = reinterpret_cast<const float4 *>(st.kd + 4 * st.nsites + d.site_off);
     cx.p64 = st.p64 + d.site_off;
     cx.b2 = st.b2[d.slot_base + slot];
-    cx.j2q = cc.j2x2 * 0.25f;
-    cx.j3q = cc.j3x2 * 0.25f;
+    cx.j2d = cc.j2x2 * 0x1p22f;
+    cx.j3d = cc.j3x2 * 0x1p22f;
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
@@ -478,7 +493,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     // ---- observables of the final state (integer sums; phi part fixed point)
     int64_t v[OBS_N];
     constexpr int PER_Z = 4 * WPH * WPH;
-    if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) {
+    if constexpr (fast_zs<WPH, NT>() >= 2) {
         ObsFast a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int c = 0; c < 4; ++c) {
             if (c & 1) observe_fast<WPH, NT, 1>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
@@ -516,7 +531,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
-        if constexpr (NT % PER_Z == 0 && ((NT / PER_Z) % 2) == 0) v[OBS_EQ] -= 4 * cc.qsum;
+        if constexpr (fast_zs<WPH, NT>() >= 2) v[OBS_EQ] -= 4 * cc.qsum;
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
         for (int k = 0; k < OBS_N; ++k) o[k] = v[k];
@@ -903,24 +918,12 @@ ResKernel resident_kernel(int32_t L)
     }
 }
 
-// 576 threads = 4 z-layers of an L = 24 or L = 48 colour plane per step (the
-// fixed-geometry path); only those two sizes are built with it.
-template <bool SHADOW>
-ResKernel resident_kernel_576(int32_t L)
-{
-    switch (L) {
-        case 24: return k_sweep_resident<3, 576, SHADOW>;
-        case 48: return k_sweep_resident<6, 576, SHADOW>;
-        default: return nullptr;
-    }
-}
-
 ResKernel pick_resident(int32_t L, int32_t nt, bool shadow)
 {
     switch (nt) {
         case 256: return shadow ? resident_kernel<256, true>(L) : resident_kernel<256, false>(L);
         case 512: return shadow ? resident_kernel<512, true>(L) : resident_kernel<512, false>(L);
-        case 576: return shadow ? resident_kernel_576<true>(L) : resident_kernel_576<false>(L);
+        case 576: return shadow ? resident_kernel<576, true>(L) : resident_kernel<576, false>(L);
         case 1024: return shadow ? resident_kernel<1024, true>(L) : resident_kernel<1024, false>(L);
         default: return nullptr;
     }

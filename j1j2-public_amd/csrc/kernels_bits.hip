@@ -117,6 +117,7 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     const BitSites s = bit_sites<HW, HX>(lds32, c, NU, u, n, m0);
     const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
     const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
+    const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
     const uint32_t xpe = f16_even(s.xp), xpo = f16_odd(s.xp), xme = f16_even(s.xm), xmo = f16_odd(s.xm);
     const uint32_t ype = f16_even(s.yp), ypo = f16_odd(s.yp), yme = f16_even(s.ym), ymo = f16_odd(s.ym);
     const int t = z * TPL + tid;                              // plane-relative group = Philox counter
@@ -139,8 +140,8 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
         h = __builtin_fmaf(f16_site(xme, xmo, k), kxma[k], h);
         h = __builtin_fmaf(f16_site(ype, ypo, k), kypa[k], h);
         h = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h);
-        h = __builtin_fmaf(byte_f(D, k), cx.j2q, h);
-        h = __builtin_fmaf(byte_f(Z, k), cx.j3q, h);
+        h = __builtin_fmaf(f16_site(de, dod, k), cx.j2d, h);
+        h = __builtin_fmaf(f16_site(ze, zod, k), cx.j3d, h);
         const bool acc = decide_f32_word(h, s.S, k, cx.b2, ra[k]);
         if constexpr (SHADOW) {
             const bool up = nib_bit(s.S, k) != 0;
@@ -294,8 +295,8 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     cx.kym4 = reinterpret_cast<const float4 *>(st.kd + 4 * st.nsites + d.site_off);
     cx.p64 = st.p64 + d.site_off;
     cx.b2 = st.b2[d.slot_base + slot];
-    cx.j2q = cc.j2x2 * 0.25f;
-    cx.j3q = cc.j3x2 * 0.25f;
+    cx.j2d = cc.j2x2 * 0x1p22f;
+    cx.j3d = cc.j3x2 * 0x1p22f;
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;

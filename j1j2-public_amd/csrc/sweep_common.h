@@ -10,7 +10,6 @@ namespace j1j2 {
 namespace {
 
 __device__ __forceinline__ uint32_t lds_bit(uint32_t w, int j) { return (w >> (8 * j + 2)) & 1u; }
-__device__ __forceinline__ float byte_f(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xffu); }
 
 
 // FP64 decision of Metro_sweep (J1J2_6tv.f90:2572-2604) with the engine's
@@ -77,7 +76,7 @@ __device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N])
 struct SweepCtx {
     const float4 *kc4, *kxp4, *kxm4, *kyp4, *kym4;
     const double *p64;     // config's FP64 field (shadow evaluation)
-    float b2, j2q, j3q;        // j2q = 2 J2 / 4, j3q = 2 J3 / 4 (exact scalings)
+    float b2, j2d, j3d;        // j2d = 2^24 (2 J2 / 4), j3d = 2^24 (2 J3 / 4) (exact scalings)
     double beta;
     uint32_t ctr1, sweep_lo, sweep_hi, key0, key1;
 };
@@ -99,6 +98,13 @@ __device__ __forceinline__ float4 ld16(const float4 *base, uint32_t byte_off)
 // f16_even moves bytes 0 and 2 up into the high bytes of the halves.
 __device__ __forceinline__ uint32_t f16_odd(uint32_t w) { return w & 0xFF00FF00u; }
 __device__ __forceinline__ uint32_t f16_even(uint32_t w) { return __builtin_amdgcn_perm(w, w, 0x020c000cu); }
+// The bytewise bond counts D = 4 (0..4) and Z = 4 (0..2) enter the same way,
+// as f16 denormals: a byte v as the low byte of a half whose high byte is 0 is
+// the f16 value v 2^-24, so against the coefficient 2^24 (2J/4) the product is
+// exactly v (2J/4) and the fma rounds as fma(v, 2J/4, h).  lo_even keeps bytes
+// 0 and 2 in place, lo_odd moves bytes 1 and 3 down; both zero the high bytes.
+__device__ __forceinline__ uint32_t lo_even(uint32_t w) { return w & 0x00FF00FFu; }
+__device__ __forceinline__ uint32_t lo_odd(uint32_t w) { return __builtin_amdgcn_perm(w, w, 0x0c030c01u); }
 __device__ __forceinline__ float f16_site(uint32_t even, uint32_t odd, int j)
 {
     const uint32_t w = (j & 1) ? odd : even;

@@ -44,6 +44,28 @@ def test_fp32_resident_matches_oracle(L, H, R, start):
     assert eng.sweep_path()[0] == 1 and n == 2
 
 
+@pytest.mark.parametrize("nt", [256, 512, 576, 1024])
+def test_resident_block_sizes_match_oracle(nt, monkeypatch):
+    """Every block size of the byte layout on every size it runs: the
+    fixed-geometry stepping (some threads idle when a step's z-layers do not
+    fill the block, e.g. L = 32 with 576 or L = 48 with 1024 threads) and the
+    general path (L = 48 with 256 or 512, L = 40 with 576), one launch per
+    size and one mixed launch."""
+    say(f"case resident block size {nt}, L in 8..48")
+    monkeypatch.setenv("J1J2_RES_THREADS", str(nt))
+    sizes = [(8, 4, 2), (16, 4, 2), (24, 6, 2), (32, 4, 2), (40, 4, 2), (48, 4, 2)]
+    groups = [J.make_params(L, H, nreplica=R, nconf=1, iconf0=1 + k, flags=LOCNEM, seed=SEED, ptmeasindex=2)
+              for k, (L, H, R) in enumerate(sizes)]
+    eng, n = run_and_compare(groups, ["hot", "str"] * 3, 2, 3)            # one mixed launch
+    assert eng.sweep_path()[0] == 1 and n == len(sizes)
+    assert f"nt{nt}" in eng.launch_plan()
+    eng.close()
+    for k, g in enumerate(groups):                                          # one size per launch
+        eng, n = run_and_compare([g], "str" if k % 2 else "hot", 2, 3)
+        assert n == 1 and f"L{g.L} nt{nt}" in eng.launch_plan()
+        eng.close()
+
+
 @pytest.mark.parametrize("L,H,R,start", [(56, 4, 3, "hot"), (64, 2, 2, "str"), (72, 4, 2, "hot"),
                                           (80, 2, 3, "hot"), (88, 4, 2, "str"), (96, 4, 3, "hot")])
 def test_fp32_resident_bits_matches_oracle(L, H, R, start):
