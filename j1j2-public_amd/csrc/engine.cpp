@@ -43,11 +43,17 @@ int fail(int code, const std::string &msg)
 
 constexpr double LOG2E = 0x1.71547652b82fep+0;
 
+// Work-list segment of a lattice range (one group, or all lattices).
+struct OrderSeg {
+    int32_t ord0 = 0, nord = 0;
+};
+
 struct Group {
     j1j2_params p;
     int32_t conf0, lat0, nlat, N;
     bool resident;             // swept by an LDS-resident kernel (observables fused)
     int kind = 0;              // 0 global-memory passes, 1 resident bytes, 2 resident bits
+    OrderSeg slab_seg[BIT_SLAB_MAX + 1];   // kind 2: work lists of the lattices split into S slabs
 };
 
 struct Timer {
@@ -62,11 +68,7 @@ struct Timer {
 struct ResLaunch {
     int32_t ord0, nord, L, max_n, nt, nlat;
     int32_t kind = 1, H = 0;   // kind 2: bit layout (nt fixed by L)
-};
-
-// Work-list segment of a lattice range (one group, or all lattices).
-struct OrderSeg {
-    int32_t ord0 = 0, nord = 0;
+    int32_t slabs = 1;         // kind 2: z-slabs per lattice (one block each)
 };
 
 // How a cycle's resident sweeps are launched.  The result of a cycle does not
@@ -79,7 +81,8 @@ struct Plan {
     {
         std::string d = concurrent ? "concurrent" : "sequential";
         for (const ResLaunch &r : launches)
-            d += " [L" + std::to_string(r.L) + " nt" + std::to_string(r.nt) + " x" + std::to_string(r.nlat) + "]";
+            d += " [L" + std::to_string(r.L) + " nt" + std::to_string(r.nt) + " x" + std::to_string(r.nlat) +
+                 (r.slabs > 1 ? " s" + std::to_string(r.slabs) : std::string()) + "]";
         return d;
     }
 };
@@ -100,6 +103,7 @@ struct Engine {
     bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
     bool all_byte = false;                     // every group on the byte-layout resident kernel
     int32_t res_nt = 256;                      // threads per block of the resident sweep (default plan)
+    int32_t bit_slabs = 1;                     // z-slabs of the bit-layout lattices (default plan)
     int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
     Plan plan;                                 // resident launches of a cycle
     std::vector<OrderSeg> group_seg;           // work-list segment of each resident group
@@ -384,7 +388,7 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
                 for (size_t i = 0; i < pl.launches.size() && r == hipSuccess; ++i) {
                     const ResLaunch &L = pl.launches[i];
                     if ((r = hipStreamWaitEvent(e.side[i], e.fork_ev, 0)) != hipSuccess) break;
-                    r = L.kind == 2 ? launch_sweep_bits(e.st, L.ord0, L.nord, L.L, L.H, e.shadow, e.side[i])
+                    r = L.kind == 2 ? launch_sweep_bits(e.st, L.ord0, L.nord, L.L, L.H, L.slabs, e.shadow, e.side[i])
                                     : launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.nt, e.shadow, e.side[i]);
                     if (r != hipSuccess) break;
                     if ((r = hipEventRecord(e.join_ev[i], e.side[i])) != hipSuccess) break;
@@ -397,7 +401,7 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
             for (const ResLaunch &L : P.launches) {
                 auto fn = [](Engine &e, const void *a) {
                     const ResLaunch &r = *static_cast<const ResLaunch *>(a);
-                    return r.kind == 2 ? launch_sweep_bits(e.st, r.ord0, r.nord, r.L, r.H, e.shadow, e.stream)
+                    return r.kind == 2 ? launch_sweep_bits(e.st, r.ord0, r.nord, r.L, r.H, r.slabs, e.shadow, e.stream)
                                        : launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.nt, e.shadow,
                                                                e.stream);
                 };
@@ -489,12 +493,16 @@ std::map<std::string, Plan> &tune_cache()
     return m;
 }
 
-ResLaunch group_launch(const Engine &E, int g, int32_t nt)
+// nt: threads per block of a byte-layout group; slabs: z-slabs of a bit-layout
+// group (its block size is fixed by L)
+ResLaunch group_launch(const Engine &E, int g, int32_t nt, int32_t slabs = 1)
 {
     const Group &G = E.groups[g];
-    ResLaunch r{E.group_seg[g].ord0, E.group_seg[g].nord, G.p.L, G.N, nt, G.nlat};
+    const OrderSeg &o = (G.kind == 2 && slabs > 1) ? G.slab_seg[slabs] : E.group_seg[g];
+    ResLaunch r{o.ord0, o.nord, G.p.L, G.N, G.kind == 2 ? 0 : nt, G.nlat};
     r.kind = G.kind;
     r.H = G.p.H;
+    r.slabs = G.kind == 2 ? slabs : 1;
     return r;
 }
 
@@ -511,15 +519,16 @@ Plan default_plan(const Engine &E)
         return P;
     }
     for (int g = 0; g < (int)E.groups.size(); ++g)
-        if (E.groups[g].resident) P.launches.push_back(group_launch(E, g, E.res_nt));
+        if (E.groups[g].resident) P.launches.push_back(group_launch(E, g, E.res_nt, E.bit_slabs));
     return P;
 }
 
 // Work list of the lattices of configs [c0, c1): configs largest first,
 // each config's replicas together, configs dealt to the 8 XCDs by least
 // accumulated work, then interleaved so that block 8k + x is XCD x's k-th
-// lattice (-1 pads the shorter XCD lists).
-std::vector<int32_t> make_order(const Engine &E, int c0, int c1)
+// lattice (-1 pads the shorter XCD lists).  slabs > 1: one entry
+// BIT_SLAB_MAX * lattice + s per slab s, a lattice's slabs together on one XCD.
+std::vector<int32_t> make_order(const Engine &E, int c0, int c1, int slabs = 1)
 {
     std::vector<int> cs;
     for (int c = c0; c < c1; ++c) cs.push_back(c);
@@ -529,7 +538,12 @@ std::vector<int32_t> make_order(const Engine &E, int c0, int c1)
     for (int c : cs) {
         const int x = (int)(std::min_element(work.begin(), work.end()) - work.begin());
         const ConfDesc &cd = E.confs[c];
-        for (int r = 0; r < cd.nrep; ++r) bins[x].push_back(cd.lat0 + r);
+        for (int r = 0; r < cd.nrep; ++r) {
+            if (slabs > 1)
+                for (int q = 0; q < slabs; ++q) bins[x].push_back((cd.lat0 + r) * BIT_SLAB_MAX + q);
+            else
+                bins[x].push_back(cd.lat0 + r);
+        }
         work[x] += (int64_t)cd.N * cd.nrep;
     }
     size_t len = 0;
@@ -640,8 +654,12 @@ int tune(Engine &E, int measure, int ncycles, int &done)
     std::vector<std::vector<int32_t>> cand(R.size());
     size_t maxc = 0;
     for (size_t i = 0; i < R.size(); ++i) {
-        if (E.groups[R[i]].kind == 2) cand[i].push_back(0);     // bit layout: block size fixed by L
-        else
+        const Group &G = E.groups[R[i]];
+        if (G.kind == 2) {                                    // bit layout: whole lattices or z-slabs
+            cand[i].push_back(1);
+            for (int32_t S : {2, 4})
+                if (bits_slab_fits(G.p.L, G.p.H, S, G.slab_seg[S].nord, E.shadow)) cand[i].push_back(S);
+        } else
             for (int32_t nt : {256, 512, 576, 1024})
                 if (resident_nt_valid(E.groups[R[i]].p.L, nt)) cand[i].push_back(nt);
         maxc = std::max(maxc, cand[i].size());
@@ -659,7 +677,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
             std::vector<size_t> pick(R.size());
             for (size_t i = 0; i < R.size(); ++i) {
                 pick[i] = std::min(r, cand[i].size() - 1);
-                P.launches.push_back(group_launch(E, R[i], cand[i][pick[i]]));
+                P.launches.push_back(group_launch(E, R[i], cand[i][pick[i]], cand[i][pick[i]]));
             }
             std::vector<float> ms;
             float phase = 0.0f;
@@ -675,7 +693,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         const float tmin = *std::min_element(best[i].begin(), best[i].end());
         size_t b = 0;
         while (best[i][b] > tmin * TUNE_TIE) ++b;
-        per.launches.push_back(group_launch(E, R[i], cand[i][b]));
+        per.launches.push_back(group_launch(E, R[i], cand[i][b], cand[i][b]));
         t_seq += best[i][b];
     }
     Plan choice = per;
@@ -755,6 +773,12 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
         HIPCHK(hipMemsetAsync(E.st.locnem_pending, 0, sizeof(int32_t) * E.confs.size(), E.stream));
     }
     HIPCHK(hipStreamSynchronize(E.stream));
+    if (E.st.slab_err) {
+        int32_t err = 0;
+        HIPCHK(hipMemcpy(&err, E.st.slab_err, sizeof(err), hipMemcpyDeviceToHost));
+        if (err)
+            return fail(J1J2_EDEVICE, "bit-layout slabs: a wait for a neighbouring slab ran out (slabs not resident together)");
+    }
     if (E.timing && (rc = collect_timers(E))) return rc;
     if (ncycles > 0) E.observed = true;
     return 0;
@@ -896,6 +920,11 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         E->res_nt = (v == 512 || v == 576 || v == 1024) ? v : 256;
         E->autotune = false;
     }
+    if (const char *sl = std::getenv("J1J2_BIT_SLABS")) {
+        // a fixed slab count for the bit-layout groups (where valid), no autotuning
+        E->bit_slabs = std::atoi(sl);
+        E->autotune = false;
+    }
     if (const char *at = std::getenv("J1J2_AUTOTUNE")) E->autotune = E->autotune && std::atoi(at) != 0;
     if (std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.kind == 1; }))
         HIPCHK(setup_resident((size_t)E->max_n));
@@ -928,6 +957,12 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         return rc;
     st.flips = nullptr;
     if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
+    st.slab_x = st.slab_obs = nullptr;
+    st.slab_err = nullptr;
+    if (bits_lds &&
+        ((rc = E->alloc(st.slab_x, (int64_t)bits_slab_x_words((int32_t)nlat))) ||
+         (rc = E->alloc(st.slab_obs, nlat * BIT_SLAB_MAX * 16)) || (rc = E->alloc(st.slab_err, 1))))
+        return rc;
     st.lats = lats; st.confs = confs; st.cconst = cconst;
     st.nlat = (int32_t)nlat; st.nconf = (int32_t)nconf; st.nsites = E->nsites;
     st.kd = kd; st.p64 = p64; st.qh = q; st.ql = q + E->nsites; st.beta = beta; st.b2 = b2;
@@ -948,9 +983,28 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
             order.insert(order.end(), seg.begin(), seg.end());
             return o;
         };
-        for (int g = 0; g < (int)E->groups.size(); ++g)
-            if (E->groups[g].resident) E->group_seg[g] = add(E->groups[g].conf0, E->groups[g].conf0 + E->groups[g].p.nconf);
+        auto add_slabs = [&](int c0, int c1, int S) {
+            const std::vector<int32_t> seg = make_order(*E, c0, c1, S);
+            OrderSeg o;
+            o.ord0 = (int32_t)order.size();
+            o.nord = (int32_t)seg.size();
+            order.insert(order.end(), seg.begin(), seg.end());
+            return o;
+        };
+        for (int g = 0; g < (int)E->groups.size(); ++g) {
+            Group &G = E->groups[g];
+            if (G.resident) E->group_seg[g] = add(G.conf0, G.conf0 + G.p.nconf);
+            if (G.kind == 2)
+                for (int S : {2, 4})
+                    if (bits_slab_valid(G.p.L, G.p.H, S)) G.slab_seg[S] = add_slabs(G.conf0, G.conf0 + G.p.nconf, S);
+        }
         if (E->all_byte) E->all_seg = add(0, (int)nconf);
+        // a forced slab count (2 or 4) only where it applies to every bit-layout group
+        if (E->bit_slabs != 2 && E->bit_slabs != 4) E->bit_slabs = 1;
+        for (const Group &G : E->groups)
+            if (G.kind == 2 && E->bit_slabs > 1 &&
+                !bits_slab_fits(G.p.L, G.p.H, E->bit_slabs, G.slab_seg[E->bit_slabs].nord, E->shadow))
+                E->bit_slabs = 1;
         int32_t *dorder;
         if ((rc = E->alloc(dorder, (int64_t)order.size()))) return rc;
         if (!order.empty() && (rc = h2d(*E, dorder, order))) return rc;

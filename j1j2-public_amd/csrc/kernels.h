@@ -55,6 +55,13 @@ struct DeviceState {
     int64_t series_cap;
     int32_t *flips;            // per slot per site (optional)
     unsigned long long *shadow;// nconf x 2
+    // bit-layout lattices split into z-slabs (one block each): per lattice
+    // BIT_SLAB_MAX slots of exchange words -- 4 planes x 2 boundary layers x
+    // L/2 unit words (a 14-bit version above bit 50) and 16 words of partial
+    // observables (a 32-bit version above 32-bit halves) -- and an error flag
+    // set when a bounded wait for a neighbour's words runs out
+    unsigned long long *slab_x, *slab_obs;
+    int32_t *slab_err;
     // run constants
     int32_t ptswapever, ptmeasindex, bins, local;
     uint32_t seed;
@@ -76,8 +83,16 @@ constexpr size_t RESIDENT_SCRATCH_LDS = sizeof(int64_t) * (1024 / 64) * 8;
 bool bits_supported(int32_t L);
 size_t bits_lds_need(int32_t L, int32_t H);
 hipError_t setup_bits(size_t max_lds);
-hipError_t launch_sweep_bits(const DeviceState &st, int32_t ord0, int32_t nord, int32_t L, int32_t H, bool shadow,
-                             hipStream_t s);
+hipError_t launch_sweep_bits(const DeviceState &st, int32_t ord0, int32_t nord, int32_t L, int32_t H, int32_t slabs,
+                             bool shadow, hipStream_t s);
+// Slabs: S > 1 splits every lattice of the launch into S z-slabs of H / S
+// (even) layers, one block each, order entries 4 * lattice + slab, the S
+// blocks of a lattice co-resident (bits_slab_fits).  The result does not
+// depend on S.
+constexpr int BIT_SLAB_MAX = 4;
+bool bits_slab_valid(int32_t L, int32_t H, int32_t slabs);
+bool bits_slab_fits(int32_t L, int32_t H, int32_t slabs, int32_t nblocks, bool shadow);
+size_t bits_slab_x_words(int32_t nlat);     // slab_x words for nlat lattices (L <= 96)
 // nt in {256, 512, 576, 1024} for every L (and L = 0, mixed).
 bool resident_nt_valid(int32_t L, int32_t nt);
 size_t resident_lds_limit();

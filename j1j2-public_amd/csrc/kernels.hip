@@ -492,7 +492,6 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 
     // ---- observables of the final state (integer sums; phi part fixed point)
     int64_t v[OBS_N];
-    constexpr int PER_Z = 4 * WPH * WPH;
     if constexpr (fast_zs<WPH, NT>() >= 2) {
         ObsFast a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int c = 0; c < 4; ++c) {

@@ -19,6 +19,17 @@ namespace {
 // four sites m0+s .. m0+s+3 (m0 = 4j, s in {-1, 0, +1}) are always bits
 // m0+1+s .. m0+4+s of the word, wrap included.  One spare dword follows the
 // last word (the two-dword reads of the last unit touch it).
+//
+// A lattice may instead be split into S z-slabs, one block each (all of them
+// resident at once): a slab holds its own layers plus one halo layer below and
+// above per plane; after colour pass c it publishes its two boundary layers of
+// plane c as exchange words in global memory, each carrying a version number,
+// and before pass c' it waits for its neighbours' words of plane c'^2 (the
+// only plane whose z-neighbours pass c' reads) with the version of that
+// plane's last update.  A slot is never overwritten before it is read: a slab
+// republishes plane c only after a pass that waits for the reader's later
+// publication.  64-bit agent-scope atomic loads and stores carry each word
+// whole, so no fences are needed.
 template <int L>
 struct BitGeo {
     static constexpr int HW = L / 2;                 // sites per half-row
@@ -26,8 +37,6 @@ struct BitGeo {
     static constexpr int TPL = G * HW;               // groups per z-layer of a plane: one thread each
     static constexpr int NT = (TPL + 63) / 64 * 64;  // threads per block (idle tail lanes)
 };
-
-size_t bits_lds_bytes(int32_t L, int32_t H) { return (size_t)32 * (L / 2) * H + 8; }
 
 // Four sites starting at bit `pos` of the unit word at dword `dw`.
 __device__ __forceinline__ uint32_t nib(const uint32_t *lds32, int dw, int pos)
@@ -40,17 +49,21 @@ __device__ __forceinline__ uint32_t nib(const uint32_t *lds32, int dw, int pos)
 // kernels.hip: byte value 4b).
 __device__ __forceinline__ uint32_t spread4(uint32_t n) { return (n * 0x00810204u) & 0x04040404u; }
 
-// Neighbour units of unit u = z*HW + yh of plane c (y parity p = ((c>>1) ^ z) & 1).
+// Neighbour units of LDS unit u of plane c (global layer zg sets the y parity
+// p = ((c>>1) ^ zg) & 1).  Whole lattice: u = zg*HW + yh, z wraps; slab: u =
+// (zl+1)*HW + yh (zl the slab-local layer), z neighbours are the next and
+// previous layers (halos at the slab's ends).
 struct BitNb {
-    int uyp, uym, uzp, uzm, p;
+    int u, uyp, uym, uzp, uzm, p;
 };
 
-template <int HW>
-__device__ __forceinline__ BitNb bit_nb(int c, int z, int yh, int H)
+template <int HW, bool SLAB>
+__device__ __forceinline__ BitNb bit_nb(int c, int zg, int zl, int yh, int H)
 {
     BitNb n;
-    const int u = z * HW + yh;
-    n.p = ((c >> 1) ^ z) & 1;
+    const int u = (SLAB ? zl + 1 : zl) * HW + yh;
+    n.u = u;
+    n.p = ((c >> 1) ^ zg) & 1;
     if (n.p) {                        // y odd: y+1 is the next unit, y-1 this one
         n.uyp = (yh == HW - 1) ? u - (HW - 1) : u + 1;
         n.uym = u;
@@ -58,8 +71,13 @@ __device__ __forceinline__ BitNb bit_nb(int c, int z, int yh, int H)
         n.uyp = u;
         n.uym = (yh == 0) ? u + (HW - 1) : u - 1;
     }
-    n.uzp = (z == H - 1) ? u - (H - 1) * HW : u + HW;
-    n.uzm = (z == 0) ? u + (H - 1) * HW : u - HW;
+    if constexpr (SLAB) {
+        n.uzp = u + HW;
+        n.uzm = u - HW;
+    } else {
+        n.uzp = (zl == H - 1) ? u - (H - 1) * HW : u + HW;
+        n.uzm = (zl == 0) ? u + (H - 1) * HW : u - HW;
+    }
     return n;
 }
 
@@ -71,13 +89,14 @@ struct BitSites {
 };
 
 // HX: x parity of the plane's sites; for even sites +x is the other plane's
-// site m and -x its site m-1, for odd sites -x is m and +x is m+1.
+// site m and -x its site m-1, for odd sites -x is m and +x is m+1.  NU: units
+// per plane in LDS.
 template <int HW, int HX>
-__device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int NU, int u, const BitNb &n,
-                                              int m0)
+__device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int NU, const BitNb &n, int m0)
 {
     const int pos0 = m0 + 1, posx = m0 + 1 + (HX ? 1 : -1);
     const int p1 = 2 * (c ^ 1) * NU, p2 = 2 * (c ^ 2) * NU, p3 = 2 * (c ^ 3) * NU;
+    const int u = n.u;
     BitSites s;
     s.S = nib(lds32, 2 * (c * NU + u), pos0);
     const uint32_t x0 = spread4(nib(lds32, p1 + 2 * u, pos0)), x1 = spread4(nib(lds32, p1 + 2 * u, posx));
@@ -99,28 +118,35 @@ __device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int 
 __device__ __forceinline__ uint32_t nib_bit(uint32_t n, int k) { return (n >> k) & 1u; }
 __device__ __forceinline__ uint32_t sp_bit(uint32_t w, int k) { return (w >> (8 * k + 2)) & 1u; }
 
+// A block's share of its lattice: global layers z0 .. z0+nz-1, NU units per
+// plane in LDS (HW*H for the whole lattice, HW*(nz+2) for a slab).
+struct Share {
+    int z0, nz, NU;
+};
+
 // One 4-site group of a colour pass of the bit layout: thread (yh, j) owns
-// group j of unit (z, yh); its flips go to the unit word with an LDS atomic
-// xor (the G threads of a unit share the word; nobody reads plane c in pass c).
-template <int L, int HX, bool SHADOW>
+// group j of the half-row (zg, yh); its flips go to the unit word with an LDS
+// atomic xor (the G threads of a unit share the word; nobody reads plane c in
+// pass c).  Philox counters and disorder planes are indexed by the global
+// layer zg, so a slab updates its sites exactly as the whole lattice would.
+template <int L, int HX, bool SHADOW, bool SLAB>
 __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
-                                          int z, int yh, int j, unsigned long long &sh_prop,
+                                          int NU, int zg, int zl, int yh, int j, unsigned long long &sh_prop,
                                           unsigned long long &sh_mis)
 {
     using Gm = BitGeo<L>;
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x, m0 = 4 * j;
-    const int NU = HW * H, nq = NU * G;
+    const int NUg = HW * H, nq = NUg * G;
     unsigned long long *lds64 = reinterpret_cast<unsigned long long *>(lds32);
-    const int u = z * HW + yh;
-    const BitNb n = bit_nb<HW>(c, z, yh, H);
-    const BitSites s = bit_sites<HW, HX>(lds32, c, NU, u, n, m0);
+    const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
+    const BitSites s = bit_sites<HW, HX>(lds32, c, NU, n, m0);
     const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
     const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
     const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
     const uint32_t xpe = f16_even(s.xp), xpo = f16_odd(s.xp), xme = f16_even(s.xm), xmo = f16_odd(s.xm);
     const uint32_t ype = f16_even(s.yp), ypo = f16_odd(s.yp), yme = f16_even(s.ym), ymo = f16_odd(s.ym);
-    const int t = z * TPL + tid;                              // plane-relative group = Philox counter
+    const int t = zg * TPL + tid;                             // plane-relative group = Philox counter
     const int own = c * nq + t;
     const uint32_t ob = (uint32_t)own * 16u;
     const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob),
@@ -153,9 +179,12 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
                                    pm1(sp_bit(s.d2p, k)), pm1(sp_bit(s.d2m, k)), pm1(sp_bit(s.zp, k)),
                                    pm1(sp_bit(s.zm, k))};
             // site indices in the colour-plane order of the disorder planes
-            const int64_t q1 = (int64_t)((c ^ 1) * NU + u) * HW, q2 = (int64_t)(c ^ 2) * NU * HW;
-            const double pv[5] = {cx.p64[4 * (int64_t)own + k], cx.p64[q2 + (int64_t)n.uyp * HW + m],
-                                  cx.p64[q2 + (int64_t)n.uym * HW + m], cx.p64[q1 + mo_p], cx.p64[q1 + mo_m]};
+            // (global units: the y neighbours sit in the same layer)
+            const int ug = zg * HW + yh;
+            const int64_t q1 = (int64_t)((c ^ 1) * NUg + ug) * HW, q2 = (int64_t)(c ^ 2) * NUg * HW;
+            const int64_t uyp = ug + (n.uyp - n.u), uym = ug + (n.uym - n.u);
+            const double pv[5] = {cx.p64[4 * (int64_t)own + k], cx.p64[q2 + uyp * HW + m],
+                                  cx.p64[q2 + uym * HW + m], cx.p64[q1 + mo_p], cx.p64[q1 + mo_m]};
             const bool a64 = decide_f64(field_f64(sv, pv, cc), up, cx.beta, ra[k]);
             sh_prop += 1;
             sh_mis += (a64 != acc);
@@ -165,30 +194,38 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     unsigned long long mask = (unsigned long long)f << (m0 + 1);
     if (j == 0) mask |= (unsigned long long)(f & 1u) << (HW + 1);     // copy of site 0
     if (j == G - 1) mask |= (unsigned long long)(f >> 3);             // copy of site HW-1
-    atomicXor(&lds64[c * NU + u], mask);
+    atomicXor(&lds64[c * NU + n.u], mask);
 }
 
-// One colour pass: every thread steps through the z-layers, two per trip (the
-// two groups are independent, which the scheduler interleaves).
-template <int L, int HX, bool SHADOW>
+// One colour pass over the block's layers.  Whole lattice: two layers per
+// trip (the two groups are independent, which the scheduler interleaves; nz
+// is even).  Slab: one per trip, within the register budget of three blocks
+// per CU, whose waves hide the latency instead.
+template <int L, int HX, bool SHADOW, bool SLAB>
 __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
-                                         unsigned long long &sh_prop, unsigned long long &sh_mis)
+                                         const Share &sh, unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
     using Gm = BitGeo<L>;
     constexpr int G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G;
-    for (int z = 0; z < H; z += 2) {                 // H is even
-        bit_group<L, HX, SHADOW>(lds32, cx, cc, c, H, z, yh, j, sh_prop, sh_mis);
-        bit_group<L, HX, SHADOW>(lds32, cx, cc, c, H, z + 1, yh, j, sh_prop, sh_mis);
+    if constexpr (SLAB) {
+        for (int zl = 0; zl < sh.nz; ++zl)
+            bit_group<L, HX, SHADOW, SLAB>(lds32, cx, cc, c, H, sh.NU, sh.z0 + zl, zl, yh, j, sh_prop, sh_mis);
+    } else {
+        for (int zl = 0; zl < sh.nz; zl += 2) {
+            bit_group<L, HX, SHADOW, SLAB>(lds32, cx, cc, c, H, sh.NU, zl, zl, yh, j, sh_prop, sh_mis);
+            bit_group<L, HX, SHADOW, SLAB>(lds32, cx, cc, c, H, sh.NU, zl + 1, zl + 1, yh, j, sh_prop, sh_mis);
+        }
     }
 }
 
 // Per-thread partial observables of the bit layout (same sums as ObsFast in
-// kernels.hip, one 4-site group per step).
-template <int L, int HX>
-__device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H, const int32_t *qh,
+// kernels.hip, one 4-site group per step; +z bonds, so a slab reads its upper
+// halo).
+template <int L, int HX, bool SLAB>
+__device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H, const Share &sh, const int32_t *qh,
                                             const int32_t *ql, ObsFast &a)
 {
     using Gm = BitGeo<L>;
@@ -196,14 +233,14 @@ __device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H,
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G, m0 = 4 * j;
-    const int NU = HW * H, nq = NU * G;
+    const int nq = HW * H * G;
     uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0;
     int32_t sxm = 0, sym = 0, el = 0;
     int64_t eh = 0;
-    for (int z = 0; z < H; ++z) {
-        const int u = z * HW + yh;
-        const BitNb n = bit_nb<HW>(c, z, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, NU, u, n, m0);
+    for (int zl = 0; zl < sh.nz; ++zl) {
+        const int zg = sh.z0 + zl;
+        const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
+        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, m0);
         const uint32_t S = spread4(s.S);
         const uint32_t sx0 = S ^ s.xp, sy0 = S ^ s.yp;
         const int pcs = __builtin_popcount(s.S);
@@ -218,7 +255,7 @@ __device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H,
         sym += n.p ? -m : m;
         const uint32_t ax = sx0 + (S ^ s.xm), ay = sy0 + (S ^ s.ym);   // bytes 0, 4, 8
         const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;             // bytes l/2 + 2
-        const uint32_t ob = (uint32_t)(c * nq + z * TPL + tid) * 16u;
+        const uint32_t ob = (uint32_t)(c * nq + zg * TPL + tid) * 16u;
         const int4 h4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(qh) + ob);
         const int4 l4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(ql) + ob);
         eh += __mul24(h4.x, (int)(e & 0xffu)) + __mul24(h4.y, (int)((e >> 8) & 0xffu)) +
@@ -230,23 +267,25 @@ __device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H,
     a.sxm += sxm; a.sym += sym; a.eh += eh; a.el += el;
 }
 
-// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) from the bit layout.
-template <int L, int HX>
-__device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, int32_t *locspin, int32_t *locnem)
+// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) from the bit layout
+// (x and y neighbours only: no halo involved).
+template <int L, int HX, bool SLAB>
+__device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, const Share &sh, int32_t *locspin,
+                                           int32_t *locnem)
 {
     using Gm = BitGeo<L>;
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G, m0 = 4 * j;
-    const int NU = HW * H, nq = NU * G;
-    for (int z = 0; z < H; ++z) {
-        const int u = z * HW + yh;
-        const BitNb n = bit_nb<HW>(c, z, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, NU, u, n, m0);
+    const int nq = HW * H * G;
+    for (int zl = 0; zl < sh.nz; ++zl) {
+        const int zg = sh.z0 + zl;
+        const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
+        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, m0);
         const uint32_t S = spread4(s.S);
         const uint32_t ax = (S ^ s.xp) + (S ^ s.xm), ay = (S ^ s.yp) + (S ^ s.ym);
-        const int dw = c * nq + z * TPL + tid;
+        const int dw = c * nq + zg * TPL + tid;
         int4 ls = reinterpret_cast<int4 *>(locspin)[dw], lv = reinterpret_cast<int4 *>(locnem)[dw];
         int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
 #pragma unroll
@@ -259,19 +298,102 @@ __device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, 
     }
 }
 
-template <int L, bool SHADOW>
-__device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &d, int lat, uint32_t *lds32,
-                                          int64_t (*red)[OBS_N])
+// ---- slab exchange words
+constexpr int XV_SHIFT = 50;                             // unit words use bits 0 .. HW+1 <= 49
+constexpr unsigned long long XW_MASK = (1ull << XV_SHIFT) - 1;
+constexpr uint32_t XV_MASK = (1u << (64 - XV_SHIFT)) - 1;
+constexpr int SPIN_MAX = 1 << 21;                        // polls before a wait gives up (seconds)
+
+__device__ __forceinline__ unsigned long long xload(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void xstore(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait until the word at p carries version `ver` in its bits from
+// `shift` up; on running out (or once another wait has) the error flag is set
+// and the wait ends, so every wave always finishes.
+__device__ unsigned long long xwait(const unsigned long long *p, int shift, uint32_t ver, int32_t *err)
+{
+    unsigned long long w = xload(p);
+    for (int it = 1; (uint32_t)(w >> shift) != ver; ++it) {
+        if ((it & 255) == 0) {
+            if (it >= SPIN_MAX) {
+                atomicOr(err, 1);
+                break;
+            }
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        w = xload(p);
+    }
+    return w;
+}
+
+// Exchange word of (lattice, slab, plane, boundary w: 0 lower layer, 1 upper), unit yh.
+template <int HW>
+__device__ __forceinline__ unsigned long long *xword(const DeviceState &st, int lat, int s, int c, int w, int yh)
+{
+    return st.slab_x + ((((int64_t)lat * BIT_SLAB_MAX + s) * 4 + c) * 2 + w) * HW + yh;
+}
+
+// After pass c: the slab's boundary layers of plane c, version ver.
+template <int HW>
+__device__ __forceinline__ void slab_publish(const DeviceState &st, const uint32_t *lds32, const Share &sh, int lat,
+                                             int s, int c, uint32_t ver)
+{
+    const int tid = threadIdx.x;
+    if (tid < 2 * HW) {
+        const int w = tid >= HW, yh = tid - w * HW;
+        const unsigned long long v =
+            reinterpret_cast<const unsigned long long *>(lds32)[c * sh.NU + (w ? sh.nz : 1) * HW + yh];
+        xstore(xword<HW>(st, lat, s, c, w, yh), (v & XW_MASK) | ((unsigned long long)ver << XV_SHIFT));
+    }
+}
+
+// The halo layers of plane c from the slabs below (its upper layer) and above
+// (its lower layer), version ver; the caller synchronises before reading them.
+template <int HW>
+__device__ __forceinline__ void slab_fetch(const DeviceState &st, uint32_t *lds32, const Share &sh, int lat, int s,
+                                           int S, int c, uint32_t ver)
+{
+    const int tid = threadIdx.x;
+    if (tid < 2 * HW) {
+        const int w = tid >= HW, yh = tid - w * HW;            // w = 0: lower halo, 1: upper
+        const int src = w ? (s + 1 == S ? 0 : s + 1) : (s == 0 ? S - 1 : s - 1);
+        const unsigned long long x = xwait(xword<HW>(st, lat, src, c, 1 - w, yh), XV_SHIFT, ver, st.slab_err);
+        reinterpret_cast<unsigned long long *>(lds32)[c * sh.NU + (w ? sh.nz + 1 : 0) * HW + yh] = x & XW_MASK;
+    }
+}
+
+template <int L, bool SHADOW, bool SLAB>
+__device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &d, int lat, int s, int S,
+                                          uint32_t *lds32, int64_t (*red)[OBS_N])
 {
     using Gm = BitGeo<L>;
     constexpr int HW = Gm::HW, G = Gm::G, NT = Gm::NT;
     const int H = d.H, tid = threadIdx.x;
-    const int NU = HW * H;
+    const int NUg = HW * H;
+    Share sh;
+    sh.nz = SLAB ? H / S : H;
+    sh.z0 = SLAB ? s * sh.nz : 0;
+    sh.NU = SLAB ? HW * (sh.nz + 2) : NUg;
     uint8_t *gsp = st.spins + d.spin_off;
 
-    // bytes (colour-plane order, unit U = c*NU + u at byte U*HW) -> unit words
-    for (int U = tid; U < 4 * NU; U += NT) {
-        const uint8_t *src = gsp + (int64_t)U * HW;
+    // bytes (colour-plane order, global unit c*NUg + z*HW + yh at byte HW times
+    // that) -> unit words; a slab takes its layers and the two halo layers
+    for (int U = tid; U < 4 * sh.NU; U += NT) {
+        const int c = U / sh.NU, r = U - c * sh.NU, li = r / HW, yh = r - li * HW;
+        int zg = li;
+        if constexpr (SLAB) {
+            zg = sh.z0 - 1 + li;
+            zg = zg < 0 ? zg + H : (zg >= H ? zg - H : zg);
+        }
+        const uint8_t *src = gsp + ((int64_t)c * NUg + (int64_t)zg * HW + yh) * HW;
         unsigned long long w = 0;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -282,7 +404,7 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
         w |= (w & 2ull) << HW;                                 // site 0 at bit HW+1
         reinterpret_cast<unsigned long long *>(lds32)[U] = w;
     }
-    if (tid == 0) lds32[8 * NU] = 0;                            // the spare dword
+    if (tid == 0) lds32[8 * sh.NU] = 0;                         // the spare dword
 
     const ConfDesc cd = st.confs[d.conf];
     const ConfConst cc = st.cconst[d.conf];
@@ -301,14 +423,22 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
     const uint64_t S0 = st.clk->cycle * (uint64_t)st.ptswapever;
+    // exchange versions count the lattice's slab launches (word 0 of slab 0's
+    // observable slots, advanced by slab 0 once every slab has published its
+    // last words): pass c of sweep k is version 4k + c + 1 after vbase, so a
+    // slot's stale version never equals the one awaited
+    unsigned long long *ox = SLAB ? st.slab_obs + (int64_t)lat * BIT_SLAB_MAX * 16 : nullptr;
+    if (SLAB && tid == 0) red[0][0] = (int64_t)xload(ox);
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
+    const uint32_t lc = SLAB ? (uint32_t)red[0][0] : 0u;
+    const uint32_t vbase = lc * (uint32_t)(4 * st.ptswapever + 1);
     if (st.fuse_locnem && slot == st.ptmeasindex - 1 && st.locnem_pending[d.conf]) {
         for (int c = 0; c < 4; ++c) {
-            if (c & 1) bit_locnem<L, 1>(lds32, c, H, st.locspin + d.site_off, st.locnem + d.site_off);
-            else bit_locnem<L, 0>(lds32, c, H, st.locspin + d.site_off, st.locnem + d.site_off);
+            if (c & 1) bit_locnem<L, 1, SLAB>(lds32, c, H, sh, st.locspin + d.site_off, st.locnem + d.site_off);
+            else bit_locnem<L, 0, SLAB>(lds32, c, H, sh, st.locspin + d.site_off, st.locnem + d.site_off);
         }
-        if (tid == 0) st.locsamples[d.conf] += 1;
+        if (tid == 0 && s == 0) st.locsamples[d.conf] += 1;
         __syncthreads();
     }
 
@@ -319,16 +449,34 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
-            if (c & 1) bit_pass<L, 1, SHADOW>(lds32, cx, cc, c, H, sh_prop, sh_mis);
-            else bit_pass<L, 0, SHADOW>(lds32, cx, cc, c, H, sh_prop, sh_mis);
+            if constexpr (SLAB) {
+                // plane c^2 was last updated in pass c^2 of this sweep (c >= 2) or
+                // of the previous one (c < 2; at k = 0 the halos just loaded)
+                if (c >= 2 || k > 0) {
+                    const uint32_t ver = vbase + 4u * (uint32_t)(c >= 2 ? k : k - 1) + (uint32_t)(c ^ 2) + 1u;
+                    slab_fetch<HW>(st, lds32, sh, lat, s, S, c ^ 2, ver & XV_MASK);
+                    __syncthreads();
+                }
+            }
+            if (c & 1) bit_pass<L, 1, SHADOW, SLAB>(lds32, cx, cc, c, H, sh, sh_prop, sh_mis);
+            else bit_pass<L, 0, SHADOW, SLAB>(lds32, cx, cc, c, H, sh, sh_prop, sh_mis);
             __syncthreads();
+            if constexpr (SLAB) slab_publish<HW>(st, lds32, sh, lat, s, c, (vbase + 4u * k + c + 1u) & XV_MASK);
         }
+    }
+    if constexpr (SLAB) {
+        // the final planes 2 and 3 of the neighbours (planes 0 and 1 were
+        // fetched after their last update); the observables read the upper halo
+        const uint32_t kl = (uint32_t)st.ptswapever - 1u;
+        slab_fetch<HW>(st, lds32, sh, lat, s, S, 2, (vbase + 4u * kl + 3u) & XV_MASK);
+        slab_fetch<HW>(st, lds32, sh, lat, s, S, 3, (vbase + 4u * kl + 4u) & XV_MASK);
+        __syncthreads();
     }
 
     ObsFast a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < 4; ++c) {
-        if (c & 1) bit_observe<L, 1>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
-        else bit_observe<L, 0>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
+        if (c & 1) bit_observe<L, 1, SLAB>(lds32, c, H, sh, st.qh + d.site_off, st.ql + d.site_off, a);
+        else bit_observe<L, 0, SLAB>(lds32, c, H, sh, st.qh + d.site_off, st.ql + d.site_off, a);
     }
     const int64_t nw = (int64_t)a.nw;
     int64_t v[OBS_N];
@@ -348,10 +496,12 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
             atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
         }
     }
-    // unit words -> bytes
-    for (int U = tid; U < 4 * NU; U += NT) {
-        const unsigned long long w = reinterpret_cast<const unsigned long long *>(lds32)[U];
-        uint8_t *dst = gsp + (int64_t)U * HW;
+    // unit words -> bytes (a slab stores its own layers)
+    const int off = SLAB ? HW : 0;
+    for (int U = tid; U < 4 * sh.nz * HW; U += NT) {
+        const int c = U / (sh.nz * HW), r = U - c * sh.nz * HW;
+        const unsigned long long w = reinterpret_cast<const unsigned long long *>(lds32)[c * sh.NU + off + r];
+        uint8_t *dst = gsp + ((int64_t)c * NUg + (int64_t)sh.z0 * HW + r) * HW;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
             const uint32_t n4 = (uint32_t)(w >> (1 + 4 * k)) & 0xFu;
@@ -360,6 +510,28 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
+        if constexpr (SLAB) {
+            // partial sums as 32-bit halves under the 32-bit version lc + 1;
+            // slab 0 adds the others' and writes the lattice's observables
+            const uint32_t ov = lc + 1u;
+            if (s != 0) {
+#pragma unroll
+                for (int k = 0; k < OBS_N; ++k) {
+                    const uint64_t u = (uint64_t)v[k];
+                    xstore(ox + s * 16 + 2 * k, ((unsigned long long)ov << 32) | (uint32_t)u);
+                    xstore(ox + s * 16 + 2 * k + 1, ((unsigned long long)ov << 32) | (uint32_t)(u >> 32));
+                }
+                return;
+            }
+            for (int q = 1; q < S; ++q)
+#pragma unroll
+                for (int k = 0; k < OBS_N; ++k) {
+                    const uint32_t lo = (uint32_t)xwait(ox + q * 16 + 2 * k, 32, ov, st.slab_err);
+                    const uint32_t hi = (uint32_t)xwait(ox + q * 16 + 2 * k + 1, 32, ov, st.slab_err);
+                    v[k] += (int64_t)(((uint64_t)hi << 32) | lo);
+                }
+            xstore(ox, (unsigned long long)(lc + 1u));   // every slab has read lc and published
+        }
         v[OBS_EQ] -= 4 * cc.qsum;
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
@@ -367,11 +539,12 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     }
 }
 
-// At most 168 VGPRs (three waves per SIMD): one lattice of L >= 64 fills most
-// of a CU's LDS, so a CU holds one or two blocks of at most 9 waves.
+// Whole lattice.  At most 168 VGPRs (three waves per SIMD): one lattice of
+// L >= 64 fills most of a CU's LDS, so a CU holds one or two blocks of at most
+// 9 waves.
 template <int L, bool SHADOW>
 __global__ __launch_bounds__(BitGeo<L>::NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 3, 8)))
-void k_sweep_bits(DeviceState st, int32_t ord0, int32_t nord, int32_t lds_bytes)
+void k_sweep_bits(DeviceState st, int32_t ord0, int32_t nord, int32_t lds_bytes, int32_t slabs)
 {
     // the lattice's unit words from LDS address 0, the reduction scratch after them
     extern __shared__ uint32_t lds32[];
@@ -379,23 +552,44 @@ void k_sweep_bits(DeviceState st, int32_t ord0, int32_t nord, int32_t lds_bytes)
     if (lat < 0) return;
     const LatDesc d = st.lats[lat];
     int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(reinterpret_cast<char *>(lds32) + lds_bytes);
-    bits_body<L, SHADOW>(st, d, lat, lds32, red);
+    bits_body<L, SHADOW, false>(st, d, lat, 0, 1, lds32, red);
 }
 
-using BitKernel = void (*)(DeviceState, int32_t, int32_t, int32_t);
+// Slabs.  At most 72 VGPRs (seven waves per SIMD), so that three slab blocks
+// of up to 9 waves share a CU.
+template <int L, bool SHADOW>
+__global__ __launch_bounds__(BitGeo<L>::NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 7, 8)))
+void k_sweep_bits_slab(DeviceState st, int32_t ord0, int32_t nord, int32_t lds_bytes, int32_t slabs)
+{
+    extern __shared__ uint32_t lds32[];
+    const int e = st.order[ord0 + blockIdx.x];
+    if (e < 0) return;
+    const int lat = e / BIT_SLAB_MAX, s = e - lat * BIT_SLAB_MAX;
+    const LatDesc d = st.lats[lat];
+    int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(reinterpret_cast<char *>(lds32) + lds_bytes);
+    bits_body<L, SHADOW, true>(st, d, lat, s, slabs, lds32, red);
+}
 
-template <bool SHADOW>
+using BitKernel = void (*)(DeviceState, int32_t, int32_t, int32_t, int32_t);
+
+template <bool SHADOW, bool SLAB>
 BitKernel bits_kernel(int32_t L)
 {
     switch (L) {
-        case 56: return k_sweep_bits<56, SHADOW>;
-        case 64: return k_sweep_bits<64, SHADOW>;
-        case 72: return k_sweep_bits<72, SHADOW>;
-        case 80: return k_sweep_bits<80, SHADOW>;
-        case 88: return k_sweep_bits<88, SHADOW>;
-        case 96: return k_sweep_bits<96, SHADOW>;
+        case 56: return SLAB ? k_sweep_bits_slab<56, SHADOW> : k_sweep_bits<56, SHADOW>;
+        case 64: return SLAB ? k_sweep_bits_slab<64, SHADOW> : k_sweep_bits<64, SHADOW>;
+        case 72: return SLAB ? k_sweep_bits_slab<72, SHADOW> : k_sweep_bits<72, SHADOW>;
+        case 80: return SLAB ? k_sweep_bits_slab<80, SHADOW> : k_sweep_bits<80, SHADOW>;
+        case 88: return SLAB ? k_sweep_bits_slab<88, SHADOW> : k_sweep_bits<88, SHADOW>;
+        case 96: return SLAB ? k_sweep_bits_slab<96, SHADOW> : k_sweep_bits<96, SHADOW>;
         default: return nullptr;
     }
+}
+
+BitKernel pick_bits(int32_t L, bool shadow, bool slab)
+{
+    return shadow ? (slab ? bits_kernel<true, true>(L) : bits_kernel<true, false>(L))
+                  : (slab ? bits_kernel<false, true>(L) : bits_kernel<false, false>(L));
 }
 
 int bits_threads(int32_t L)
@@ -413,31 +607,62 @@ int bits_threads(int32_t L)
 
 constexpr size_t BITS_RED_LDS = sizeof(int64_t) * (1024 / 64) * OBS_N;
 
+// LDS of a block: its unit words and the spare dword, rounded to 8 bytes
+size_t bits_block_lds(int32_t L, int32_t H, int32_t slabs)
+{
+    const size_t words = (size_t)4 * (L / 2) * (slabs > 1 ? H / slabs + 2 : H);
+    return (8 * words + 4 + 7) / 8 * 8;
+}
+
 }  // namespace
 
 bool bits_supported(int32_t L) { return L > 48 && L <= 96 && (L % 8) == 0; }
 
-size_t bits_lds_need(int32_t L, int32_t H) { return (bits_lds_bytes(L, H) + 7) / 8 * 8 + BITS_RED_LDS; }
+size_t bits_lds_need(int32_t L, int32_t H) { return bits_block_lds(L, H, 1) + BITS_RED_LDS; }
+
+bool bits_slab_valid(int32_t L, int32_t H, int32_t slabs)
+{
+    return bits_supported(L) && slabs > 1 && slabs <= BIT_SLAB_MAX && H % (2 * slabs) == 0;
+}
+
+bool bits_slab_fits(int32_t L, int32_t H, int32_t slabs, int32_t nblocks, bool shadow)
+{
+    if (!bits_slab_valid(L, H, slabs)) return false;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    const size_t lds = bits_block_lds(L, H, slabs) + BITS_RED_LDS;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)pick_bits(L, shadow, true),
+                                                     bits_threads(L), lds) != hipSuccess)
+        return false;
+    return (int64_t)per * ncu >= nblocks;
+}
+
+size_t bits_slab_x_words(int32_t nlat) { return (size_t)nlat * BIT_SLAB_MAX * 4 * 2 * 48; }
 
 hipError_t setup_bits(size_t max_lds)
 {
     for (int L : {56, 64, 72, 80, 88, 96})
-        for (bool sh : {false, true}) {
-            hipError_t e = hipFuncSetAttribute((const void *)(sh ? bits_kernel<true>(L) : bits_kernel<false>(L)),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
-            if (e != hipSuccess) return e;
-        }
+        for (bool sh : {false, true})
+            for (bool sl : {false, true}) {
+                hipError_t e = hipFuncSetAttribute((const void *)pick_bits(L, sh, sl),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_lds);
+                if (e != hipSuccess) return e;
+            }
     return hipSuccess;
 }
 
-hipError_t launch_sweep_bits(const DeviceState &st, int32_t ord0, int32_t nord, int32_t L, int32_t H, bool shadow,
-                             hipStream_t s)
+hipError_t launch_sweep_bits(const DeviceState &st, int32_t ord0, int32_t nord, int32_t L, int32_t H, int32_t slabs,
+                             bool shadow, hipStream_t s)
 {
     if (nord <= 0) return hipSuccess;
-    const BitKernel k = shadow ? bits_kernel<true>(L) : bits_kernel<false>(L);
+    const bool slab = slabs > 1;
+    if (slab && !bits_slab_valid(L, H, slabs)) return hipErrorInvalidValue;
+    const BitKernel k = pick_bits(L, shadow, slab);
     if (!k) return hipErrorInvalidValue;
-    const int32_t lds = (int32_t)((bits_lds_bytes(L, H) + 7) / 8 * 8);
-    hipLaunchKernelGGL(k, dim3(nord), dim3(bits_threads(L)), (size_t)lds + BITS_RED_LDS, s, st, ord0, nord, lds);
+    const int32_t lds = (int32_t)bits_block_lds(L, H, slab ? slabs : 1);
+    hipLaunchKernelGGL(k, dim3(nord), dim3(bits_threads(L)), (size_t)lds + BITS_RED_LDS, s, st, ord0, nord, lds,
+                       slab ? slabs : 1);
     return hipGetLastError();
 }
 

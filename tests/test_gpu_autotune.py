@@ -64,3 +64,17 @@ def test_autotuned_single_size_matches_oracle():
     ora.run(20, True)
     compare_state(eng, 2, ora, J.ladder(gg))
     eng.close()
+
+
+def test_autotuned_bit_layout_is_bit_identical():
+    """Bit-layout candidates: whole lattices, 2 or 4 z-slabs per lattice."""
+    say("case autotune bits L=64 H=16")
+    groups = [J.make_params(64, 16, nreplica=4, nconf=2, iconf0=1, seed=SEED, flags=J.FLAG_WRITELOCNEM)]
+    eng_t, inputs, plan_t, st_t = run(groups, ["hot", "str"], True, CYCLES)
+    say(f"  tuned plan: {plan_t}")
+    assert "autotuned" in plan_t
+    eng_t.close()
+    eng_d, _, plan_d, st_d = run(groups, ["hot", "str"], False, CYCLES)
+    assert "default" in plan_d and " s" not in plan_d
+    eng_d.close()
+    same(st_t, st_d)

@@ -62,14 +62,18 @@ def test_uneven_l16_to_48():
     check_properties(eng, inputs, range(8))
 
 
-@pytest.mark.parametrize("flags,path", [(0, 3), (J.FLAG_FORCE_GLOBAL, 2)])
-def test_large_l96(flags, path):
-    """L = 96, 24 replicas x 8 configs: the bit-layout resident path and the
+@pytest.mark.parametrize("flags,path,slabs", [(0, 3, 1), (0, 3, 4), (J.FLAG_FORCE_GLOBAL, 2, 1)])
+def test_large_l96(flags, path, slabs, monkeypatch):
+    """L = 96, 24 replicas x 8 configs: the bit-layout resident path (whole
+    lattices, and four z-slabs per lattice on 768 blocks) and the
     global-memory path."""
-    say(f"case L=96 24x8 path {path}")
+    say(f"case L=96 24x8 path {path} slabs {slabs}")
+    monkeypatch.setenv("J1J2_BIT_SLABS", str(slabs))
     g = J.make_params(96, 96, nreplica=24, nconf=8, seed=SEED, flags=flags)
     eng, inputs = build_engine([g], "str")
     assert eng.sweep_path()[0] == path
+    if path == 3:
+        assert (f"s{slabs}" in eng.launch_plan()) == (slabs > 1)
     eng.run(J.PHASE_EQUIL, 1)
     check_properties(eng, inputs, [0, 7])
     gg, iconf, p, s = inputs[3]

@@ -77,6 +77,30 @@ def test_fp32_resident_bits_matches_oracle(L, H, R, start):
     assert eng.sweep_path()[0] == 3 and n == 2
 
 
+@pytest.mark.parametrize("L,H,R,slabs,start", [(56, 8, 3, 4, "hot"), (64, 4, 2, 2, "str"), (72, 8, 2, 2, "hot"),
+                                                (96, 8, 2, 4, "str")])
+def test_fp32_bits_slabs_match_oracle(L, H, R, slabs, start, monkeypatch):
+    """Bit-layout lattices split into z-slabs (one block each, halo layers
+    exchanged through versioned words): bit-identical to the oracle."""
+    say(f"case fp32 bits slabs L={L} H={H} R={R} S={slabs} {start}")
+    monkeypatch.setenv("J1J2_BIT_SLABS", str(slabs))
+    g = J.make_params(L, H, nreplica=R, nconf=2, iconf0=7, precision=J.FP32_BATCHED, flags=LOCNEM,
+                      seed=SEED, ptmeasindex=min(2, R))
+    eng, n = run_and_compare([g], start, 3, 4)
+    assert eng.sweep_path()[0] == 3 and n == 2
+    assert f"s{slabs}" in eng.launch_plan()
+
+
+def test_bits_slabs_shadow_matches_oracle(monkeypatch):
+    """Slabs with the FP64 shadow decisions counted, one cycle per swap."""
+    say("case bits slabs + shadow")
+    monkeypatch.setenv("J1J2_BIT_SLABS", "2")
+    g = J.make_params(64, 8, nreplica=2, nconf=2, iconf0=2, ptswapever=1,
+                      flags=J.FLAG_SHADOW_FP64 | LOCNEM, seed=SEED)
+    eng, n = run_and_compare([g], "hot", 2, 3, shadow=True)
+    assert n == 2 and "s2" in eng.launch_plan()
+
+
 def test_bits_shadow_and_mixed_with_bytes():
     """Bit and byte layouts in one engine (separate launches), with the FP64
     shadow decisions counted on both, against the oracle."""
