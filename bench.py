@@ -210,11 +210,11 @@ WORKLOADS = {
 }
 
 
-def other_workload(J, name, steps, warmup):
+def other_workload(J, name, steps, warmup, with_fp64=True):
     desc, groups = WORKLOADS[name]
     fp32 = time_groups(J, groups, J.FP32_BATCHED, steps, warmup)
     n64 = max(steps // 10, 5)
-    fp64 = time_groups(J, groups, J.FP64_PLAIN, n64, max(warmup // 10, 2))
+    fp64 = time_groups(J, groups, J.FP64_PLAIN, n64, max(warmup // 10, 2)) if with_fp64 else None
     su_launch = fp32["su"] / max(fp32["launches"], 1)
     avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)
     rate, n = group_mismatch(J, groups, 4)
@@ -227,7 +227,8 @@ def other_workload(J, name, steps, warmup):
         "sweep_kernel": {"avg_launch_ms": avg_s * 1e3, "launches_per_step": fp32["launches"] / steps,
                          "algorithmic_GBps": su_launch * fp32["bytes_per"] / avg_s / 1e9,
                          "frac_of_hbm_peak": su_launch * fp32["bytes_per"] / avg_s / HBM_PEAK},
-        "fp64_plain": {"value": fp64["rate"], "steps": n64, "fp32_over_fp64": fp32["rate"] / fp64["rate"]},
+        "fp64_plain": ({"value": fp64["rate"], "steps": n64, "fp32_over_fp64": fp32["rate"] / fp64["rate"]}
+                       if fp64 else None),
         "flip_mismatch_rate": {"fp32_vs_fp64": rate, "proposals": n},
     }
 
@@ -261,7 +262,7 @@ def main():
         del src, dst
 
     if args.workload != "c3":
-        print(json.dumps(other_workload(J, args.workload, args.steps, args.warmup)), flush=True)
+        print(json.dumps(other_workload(J, args.workload, args.steps, args.warmup, not args.no_fp64)), flush=True)
         return
 
     fp32 = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=True)
