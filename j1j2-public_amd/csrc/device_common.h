@@ -169,16 +169,46 @@ __device__ __forceinline__ float threshold32(float y)
     return ldexpf(p, (int)n);
 }
 
-// Metropolis decision of the FP32 rule for the site whose spin is bit `bit`
-// of the word `sw` (set for s = +1), local field h, b2 = 2 beta log2(e), and
-// the site's Philox word r: y = -s b2 h, accept iff (float)r <= 2^32 2^y.
-// A move with y >= 0 (dE <= 0) is always taken: its threshold is >= 2^32, the
-// largest value (float)r rounds to.
-__device__ __forceinline__ bool decide_f32_word(float h, uint32_t sw, int bit, float b2, uint32_t r)
+// Exponent of the FP32 rule for the site whose spin is bit `bit` of the word
+// `sw` (set for s = +1), local field h, b2 = 2 beta log2(e): y = -s b2 h.
+__device__ __forceinline__ float y_f32_word(float h, uint32_t sw, int bit, float b2)
 {
     const float g = b2 * h;
     const uint32_t sgn = (sw << (31 - bit)) & 0x80000000u;
-    const float y = __uint_as_float(__float_as_uint(g) ^ sgn);
+    return __uint_as_float(__float_as_uint(g) ^ sgn);
+}
+
+// Metropolis decision of the FP32 rule with the site's Philox word r: accept
+// iff (float)r <= 2^32 2^y.  A move with y >= 0 (dE <= 0) is always taken:
+// its threshold is >= 2^32, the largest value (float)r rounds to.
+__device__ __forceinline__ bool decide_f32_word(float h, uint32_t sw, int bit, float b2, uint32_t r)
+{
+    return (float)r <= threshold32(y_f32_word(h, sw, bit, b2));
+}
+
+// The same decision, first through the hardware exp2 (v_exp_f32) as a filter.
+// Both 2^32 v_exp(y) and threshold32(y) lie within a relative 2^-20 of
+// 2^32 2^y wherever the threshold is a normal float (the polynomial within
+// 1.6e-8; the hardware exp2 within a few ulp), so (float)r below
+// 2^32 (1 - 2^-16) v_exp(y) is accepted by the exact rule and above
+// 2^32 (1 + 2^-16) v_exp(y) rejected by it.  Below the normal range both
+// bounds are < 1 <= (float)r (rejected either way) unless r = 0, which is
+// left open like every (float)r inside the band: those sites (a fraction
+// ~2^-15 of the proposals) take the exact rule, decide_f32_open.
+struct Filt {
+    bool acc, open;
+};
+
+__device__ __forceinline__ Filt decide_f32_filter(float y, uint32_t r)
+{
+    const float ur = (float)r, e = __builtin_amdgcn_exp2f(y);
+    const bool acc = ur < e * 0x1.fffep31f;          // 2^32 (1 - 2^-16)
+    const bool rej = ur > e * 0x1.0001p32f;          // 2^32 (1 + 2^-16)
+    return Filt{acc, !(acc || rej)};
+}
+
+__device__ __forceinline__ bool decide_f32_open(float y, uint32_t r)
+{
     return (float)r <= threshold32(y);
 }
 

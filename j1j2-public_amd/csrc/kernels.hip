@@ -219,14 +219,15 @@ __device__ __forceinline__ void x_from(uint32_t A, uint32_t B, uint32_t &Xp, uin
 // q_i e_i with e_i = l_i / 2 + 2 in 0..4 (sum q_i l_i = 2 sum q_i e_i -
 // 4 sum q_i, the last sum being a per-config constant).
 
-// The fixed-geometry stepping: a step covers ZS z-layers of a colour plane
-// (NT / PER_Z rounded down to even, so each thread's y parity stays fixed);
-// threads ZS * PER_Z.. of the block idle through the pass.  Used when ZS >= 2
-// and at least 3/4 of the threads work (0 otherwise: the general path).
+// The fixed-geometry stepping: a step covers ZS = NT / PER_Z z-layers of a
+// colour plane and threads ZS * PER_Z.. of the block idle through the pass.
+// An even ZS keeps each thread's y parity (one set of neighbour offsets); an
+// odd ZS alternates it from step to step (two sets, used in turn).  Used when
+// ZS >= 2 and at least 3/4 of the threads work (0 otherwise: the general path).
 template <int WPH, int NT>
 constexpr int fast_zs()
 {
-    constexpr int PER_Z = 4 * WPH * WPH, ZS = (NT / PER_Z) & ~1;
+    constexpr int PER_Z = 4 * WPH * WPH, ZS = NT / PER_Z;
     return (ZS >= 2 && 4 * ZS * PER_Z >= 3 * NT) ? ZS : 0;
 }
 
@@ -242,13 +243,20 @@ __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H
     if constexpr (PER_Z % 64 == 0) z0 = __builtin_amdgcn_readfirstlane(z0);
     const int u0 = z0 * HALF + tz / WPH;
     const UnitGeo g = unit_geo<WPH>(u0, c, H);
-    const NbOff o = nb_offsets<WPH, HX>(c, nq, u0, w, g);
+    const NbOff oa = nb_offsets<WPH, HX>(c, nq, u0, w, g);
+    // odd ZSTEP: every other step is in the other y parity (offsets ob)
+    const int u1 = u0 + (ZSTEP & 1) * ZSTEP * HALF;
+    const UnitGeo g1 = unit_geo<WPH>(u1, c, H);
+    const NbOff ob = nb_offsets<WPH, HX>(c, nq, u1, w, g1);
     const int dY = ((c ^ 2) - c) * nq;
     const int own0 = c * nq + z0 * PER_Z + tz;
-    uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0;
+    uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0, ps1 = 0, nw1 = 0;
     int64_t eh = 0;
     int32_t el = 0;
+#pragma unroll 2
     for (int z = z0; z < H; z += ZSTEP) {
+        const bool alt = (ZSTEP & 1) && (((z - z0) / ZSTEP) & 1);
+        const NbOff &o = alt ? ob : oa;
         const int own = own0 + (z - z0) * PER_Z;
         const uint32_t S = lds32[own];
         const uint32_t YP = lds32[own + o.yp], YM = lds32[own + o.ym];
@@ -257,12 +265,17 @@ __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H
         x_from<HX>(lds32[own + o.xa], lds32[own + o.xb], Xp0, Xm0);
         x_from<HX>(lds32[own + o.d1a], lds32[own + o.d1b], Xp1, Xm1);
         const uint32_t sx0 = S ^ Xp0, sy0 = S ^ YP;
-        ps += __builtin_popcount(S);
+        if (alt) {
+            ps1 += __builtin_popcount(S);
+            nw1 += 1;
+        } else {
+            ps += __builtin_popcount(S);
+            nw += 1;
+        }
         px += __builtin_popcount(sx0);
         py += __builtin_popcount(sy0);
         pz += __builtin_popcount(S ^ ZP);
         pd += __builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1);
-        nw += 1;
         const uint32_t ax = sx0 + (S ^ Xm0), ay = sy0 + (S ^ YM);          // bytes 0, 4, 8
         const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;                 // bytes l/2 + 2
         const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
@@ -274,10 +287,10 @@ __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H
               __mul24(l4.z, (int)((e >> 16) & 0xffu)) + __mul24(l4.w, (int)(e >> 24));
         eh += sh;
     }
-    const int32_t m = 2 * (int32_t)ps - 4 * (int32_t)nw;
-    a.ps += ps; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += nw;
-    a.sxm += HX ? -m : m;
-    a.sym += g.p ? -m : m;
+    const int32_t m = 2 * (int32_t)ps - 4 * (int32_t)nw, m1 = 2 * (int32_t)ps1 - 4 * (int32_t)nw1;
+    a.ps += ps + ps1; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += nw + nw1;
+    a.sxm += HX ? -(m + m1) : m + m1;
+    a.sym += (g.p ? -m : m) + (g1.p ? -m1 : m1);
     a.eh += eh;
     a.el += el;
 }
@@ -349,6 +362,8 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
     const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
     uint32_t flip = 0;
+    float ys[4];
+    uint32_t open = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         float h = kca[j];
@@ -358,7 +373,14 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
         h = __builtin_fmaf(f16_site(yme, ymo, j), kyma[j], h);
         h = __builtin_fmaf(f16_site(de, dod, j), cx.j2d, h);  // (2^-24 4D) (2^24 2 J2 / 4)
         h = __builtin_fmaf(f16_site(ze, zod, j), cx.j3d, h);
-        const bool acc = decide_f32_word(h, S, 8 * j + 2, cx.b2, ra[j]);
+        ys[j] = y_f32_word(h, S, 8 * j + 2, cx.b2);
+        if constexpr (!SHADOW) {
+            const Filt f = decide_f32_filter(ys[j], ra[j]);
+            flip |= (f.acc ? 4u : 0u) << (8 * j);
+            open |= f.open ? 1u << j : 0u;
+            continue;
+        }
+        const bool acc = decide_f32_open(ys[j], ra[j]);
         if constexpr (SHADOW) {
             const bool up = lds_bit(S, j) != 0;
             const int m = 4 * w + j;                          // position in the half-row
@@ -377,6 +399,11 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
         }
         flip |= (acc ? 4u : 0u) << (8 * j);
     }
+    if (!SHADOW && open) {                                // rare: the exact rule for the open sites
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (((open >> j) & 1u) && decide_f32_open(ys[j], ra[j])) flip |= 4u << (8 * j);
+    }
     *P.own = S ^ flip;
 }
 
@@ -388,11 +415,13 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH;    // dwords per z-layer of a plane
     const int nq = PER_Z * H, tid = threadIdx.x;
     if constexpr (fast_zs<WPH, NT>() >= 2) {
-        // Each thread keeps its (y/2, w) and steps z by ZSTEP (even, so y's
-        // parity is fixed): every neighbour offset is constant except at the
-        // z boundary.  Four steps per trip: a later step's LDS addresses are
-        // the first's plus a constant, which the reads take as an immediate.
+        // Each thread keeps its (y/2, w) and steps z by ZSTEP: every neighbour
+        // offset is constant except at the z boundary (for an odd ZSTEP there
+        // are two sets, P for the steps in z0's y parity and P1 for the
+        // others).  Four steps per trip: a later step's LDS addresses are an
+        // earlier one's plus a constant, which the reads take as an immediate.
         constexpr int ZSTEP = fast_zs<WPH, NT>(), STEP = ZSTEP * PER_Z, TRIP = 4;
+        constexpr bool ODD = ZSTEP & 1;
         if (tid >= STEP) return;
         const int tz = tid % PER_Z, w = tz % WPH;
         int z0 = tid / PER_Z;
@@ -405,18 +434,29 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
         const int own0 = c * nq + z0 * PER_Z + tz;
         o.zp = dY + PER_Z;                                // away from the z boundary
         o.zm = dY - PER_Z;
-        NbPtr P = nb_ptrs(lds32, own0, o);
+        NbPtr P = nb_ptrs(lds32, own0, o), P1 = P;
         // materialise each address in one VGPR (otherwise the plane base is
         // re-added to every address on every trip)
         opaque(P.own); opaque(P.yp); opaque(P.ym); opaque(P.zp); opaque(P.zm); opaque(P.xa); opaque(P.xb);
         opaque(P.d1a); opaque(P.d1b); opaque(P.d2a); opaque(P.d2b);
+        if constexpr (ODD) {
+            // the second step's unit (z0 + ZSTEP, the other y parity); only
+            // dereferenced when that step exists
+            const int u1 = u0 + ZSTEP * HALF;
+            NbOff o1 = nb_offsets<WPH, HX>(c, nq, u1, w, unit_geo<WPH>(u1, c, H));
+            o1.zp = dY + PER_Z;
+            o1.zm = dY - PER_Z;
+            P1 = nb_ptrs(lds32, own0 + STEP, o1);
+            opaque(P1.own); opaque(P1.yp); opaque(P1.ym); opaque(P1.zp); opaque(P1.zm); opaque(P1.xa);
+            opaque(P1.xb); opaque(P1.d1a); opaque(P1.d1b); opaque(P1.d2a); opaque(P1.d2b);
+        }
         for (int k = 0; k < nk; k += TRIP) {
 #pragma unroll
             for (int i = 0; i < TRIP; ++i) {
                 const int kk = k + i;
                 if (kk < nk) {
                     const int z = z0 + kk * ZSTEP;
-                    NbPtr Q = nb_shift(P, i * STEP);
+                    NbPtr Q = (ODD && (i & 1)) ? nb_shift(P1, (i - 1) * STEP) : nb_shift(P, i * STEP);
                     if (z == H - 1) Q.zp -= H * PER_Z;     // periodic in z
                     if (z == 0) Q.zm += H * PER_Z;
                     update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q, sh_prop,
@@ -424,6 +464,7 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
                 }
             }
             P = nb_shift(P, TRIP * STEP);
+            if constexpr (ODD) P1 = nb_shift(P1, TRIP * STEP);
         }
     } else {
         for (int t = tid; t < nq; t += NT) {

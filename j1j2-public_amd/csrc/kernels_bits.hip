@@ -158,7 +158,8 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
     const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
     const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
-    uint32_t f = 0;
+    uint32_t f = 0, open = 0;
+    float ys[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         float h = kca[k];
@@ -168,7 +169,14 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
         h = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h);
         h = __builtin_fmaf(f16_site(de, dod, k), cx.j2d, h);
         h = __builtin_fmaf(f16_site(ze, zod, k), cx.j3d, h);
-        const bool acc = decide_f32_word(h, s.S, k, cx.b2, ra[k]);
+        ys[k] = y_f32_word(h, s.S, k, cx.b2);
+        if constexpr (!SHADOW) {
+            const Filt fl = decide_f32_filter(ys[k], ra[k]);
+            f |= (fl.acc ? 1u : 0u) << k;
+            open |= fl.open ? 1u << k : 0u;
+            continue;
+        }
+        const bool acc = decide_f32_open(ys[k], ra[k]);
         if constexpr (SHADOW) {
             const bool up = nib_bit(s.S, k) != 0;
             const int m = m0 + k;                             // position in the half-row
@@ -190,6 +198,11 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
             sh_mis += (a64 != acc);
         }
         f |= (acc ? 1u : 0u) << k;
+    }
+    if (!SHADOW && open) {                                // rare: the exact rule for the open sites
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (((open >> k) & 1u) && decide_f32_open(ys[k], ra[k])) f |= 1u << k;
     }
     unsigned long long mask = (unsigned long long)f << (m0 + 1);
     if (j == 0) mask |= (unsigned long long)(f & 1u) << (HW + 1);     // copy of site 0

@@ -47,13 +47,15 @@ def test_fp32_resident_matches_oracle(L, H, R, start):
 @pytest.mark.parametrize("nt", [256, 512, 576, 1024])
 def test_resident_block_sizes_match_oracle(nt, monkeypatch):
     """Every block size of the byte layout on every size it runs: the
-    fixed-geometry stepping (some threads idle when a step's z-layers do not
-    fill the block, e.g. L = 32 with 576 or L = 48 with 1024 threads) and the
-    general path (L = 48 with 256 or 512, L = 40 with 576), one launch per
+    fixed-geometry stepping with even and odd layer steps (some threads idle
+    when a step's layers do not fill the block, e.g. L = 48 with 1024
+    threads) and the general path (L = 48 with 256 threads), one launch per
     size and one mixed launch."""
     say(f"case resident block size {nt}, L in 8..48")
     monkeypatch.setenv("J1J2_RES_THREADS", str(nt))
-    sizes = [(8, 4, 2), (16, 4, 2), (24, 6, 2), (32, 4, 2), (40, 4, 2), (48, 4, 2)]
+    # H large enough that odd steps (7 layers of L = 24 at 256 threads, 9 of
+    # L = 32 at 576, 5 of L = 40, 3 and 7 of L = 48) reach their second parity
+    sizes = [(8, 4, 2), (16, 4, 2), (24, 16, 2), (32, 20, 2), (40, 12, 2), (48, 16, 2)]
     groups = [J.make_params(L, H, nreplica=R, nconf=1, iconf0=1 + k, flags=LOCNEM, seed=SEED, ptmeasindex=2)
               for k, (L, H, R) in enumerate(sizes)]
     eng, n = run_and_compare(groups, ["hot", "str"] * 3, 2, 3)            # one mixed launch

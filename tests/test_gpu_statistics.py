@@ -7,6 +7,9 @@
    reference's single run being assigned the engine's single-run spread s.
 2. FP32 batched form against the FP64 form at L = 32, 16 replicas x 64 disorder
    configs: paired per-config differences, z = mean / (std / sqrt(64)).
+3. Both forms against exact enumeration of the 2x2x4 lattice (2^16 states):
+   64 independent LOCAL chains at one temperature, z = (mean of the chains'
+   time averages - exact) / (std / sqrt(64)).
 
 Every z-score must satisfy |z| < 4 and at least 75% (resp. 85%) of them |z| < 2.
 """
@@ -137,3 +140,37 @@ def test_fp32_matches_fp64_l32():
     zs.extend((d.mean(0) / (d.std(0, ddof=1) / np.sqrt(C))).tolist())
     frac = zsummary(zs, "fp32_vs_fp64_l32")
     assert np.all(np.abs(zs) < 4.0) and frac >= 0.85
+
+
+@pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
+def test_engine_matches_exact_enumeration_2x2x4(precision):
+    """Equilibrium averages of the engine's chains on the 2x2x4 lattice against
+    the exact Boltzmann averages (oracle enumeration of all 2^16 states)."""
+    L, H, T = 2, 4, 4.0
+    say(f"case exact enumeration 2x2x4, precision {precision}")
+    rng = np.random.default_rng(11)
+    p = rng.uniform(-1.0, 1.0, L * L * H)
+    exact = O.OracleLattice(L, H, p).enumerate(1.0 / T)
+    neq, nmess = 500, 4000
+    g = J.make_params(L, H, nreplica=1, ptswapever=1, sampler=J.SAMPLER_LOCAL, ladder=J.LADDER_LOCAL,
+                      tmin=T, tmax=T, dt=1.0, nconf=NCHAIN, neq=neq, nmess=nmess, precision=precision,
+                      seed=2718)
+    eng = J.Engine([g])
+    for c in range(NCHAIN):
+        eng.upload_disorder(c, p)
+        eng.set_spins(c, J.initial_spins(L, H, J.START_HOT, 0, 1 + c))
+    eng.run(J.PHASE_EQUIL, neq)
+    eng.run(J.PHASE_MEAS, nmess)
+    means = []
+    for c in range(NCHAIN):
+        m, e, sx, sy, nb = eng.series(c, 0)[0]
+        means.append([e.mean(), (e * e).mean(), np.abs(m).mean(), (m * m).mean(), nb.mean(),
+                      (sx * sx).mean(), (sy * sy).mean()])
+    eng.close()
+    means = np.array(means)
+    ref = exact[[0, 1, 2, 3, 4, 5, 6]]
+    z = (means.mean(0) - ref) / (means.std(0, ddof=1) / np.sqrt(NCHAIN) + 1e-300)
+    for name, mu, r, zz in zip(["e", "e2", "|m|", "m2", "nbar", "sx2", "sy2"], means.mean(0), ref, z):
+        say(f"  {name:5s} engine {mu:.6f} exact {r:.6f} z {zz:.2f}")
+    zsummary(z, f"enumeration_2x2x4_{'fp32' if precision == J.FP32_BATCHED else 'fp64'}")
+    assert np.all(np.abs(z) < 4.0)
