@@ -19,6 +19,7 @@ struct LatDesc {          // one replica lattice
     int32_t conf;         // global config index
     int32_t rep;          // replica index within its config
     int32_t slot_base;    // index of the config's slot 0 in per-slot arrays
+    int32_t packed;       // spins in HBM: 0 one byte per site, 1 one bit per site
     int64_t spin_off;     // byte offset of this lattice's spins
     int64_t site_off;     // offset (in sites) of the config's per-site planes
 };

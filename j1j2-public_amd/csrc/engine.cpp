@@ -892,8 +892,9 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
                 LatDesc d{};
                 d.L = p.L; d.H = p.H; d.N = G.N; d.conf = (int32_t)E->confs.size(); d.rep = r;
                 d.slot_base = cd.slot_base; d.spin_off = E->spin_bytes; d.site_off = cd.site_off;
+                d.packed = G.kind == 1 ? 1 : 0;       // the byte-layout resident sweep keeps one bit per site
                 E->lats.push_back(d);
-                E->spin_bytes += ((int64_t)G.N + 255) / 256 * 256;
+                E->spin_bytes += ((int64_t)(d.packed ? G.N / 8 : G.N) + 255) / 256 * 256;
                 ++nlat;
             }
             E->nslots += p.nreplica;
@@ -1143,11 +1144,18 @@ int j1j2_set_spins(void *handle, int32_t conf, const int8_t *spins, int32_t nrep
     for (int64_t i = 0; i < (int64_t)nrep * cd.N; ++i)
         if (spins[i] != 1 && spins[i] != -1) return fail(J1J2_EINVAL, "spins must be +1 or -1");
     HIPCHK(hipSetDevice(E->device));
-    std::vector<uint8_t> buf(cd.N);
+    std::vector<uint8_t> buf(cd.N), bits((size_t)cd.N / 8 + 1);
     for (int r = 0; r < nrep; ++r) {
+        const LatDesc &d = E->lats[cd.lat0 + r];
         to_internal(cd.L, cd.H, spins + (int64_t)r * cd.N, buf.data());
-        HIPCHK(hipMemcpyAsync(E->st.spins + E->lats[cd.lat0 + r].spin_off, buf.data(), cd.N,
-                              hipMemcpyHostToDevice, E->stream));
+        const uint8_t *src = buf.data();
+        if (d.packed) {                                   // site i at bit i & 7 of byte i >> 3
+            std::fill(bits.begin(), bits.end(), 0);
+            for (int64_t i = 0; i < cd.N; ++i) bits[i >> 3] |= (uint8_t)(buf[i] << (i & 7));
+            src = bits.data();
+        }
+        HIPCHK(hipMemcpyAsync(E->st.spins + d.spin_off, src, d.packed ? cd.N / 8 : cd.N, hipMemcpyHostToDevice,
+                              E->stream));
         HIPCHK(hipStreamSynchronize(E->stream));
     }
     if ((rc = reset_config(*E, conf))) return rc;
@@ -1192,11 +1200,14 @@ int j1j2_get_spins(void *handle, int32_t conf, int8_t *spins, int32_t nrep)
     const ConfDesc &cd = E->confs[conf];
     if (!spins || nrep != cd.nrep) return fail(J1J2_EINVAL, "spins must hold NREPLICA*L*L*H values");
     HIPCHK(hipSetDevice(E->device));
-    std::vector<uint8_t> buf(cd.N);
+    std::vector<uint8_t> buf(cd.N), bits((size_t)cd.N / 8 + 1);
     for (int r = 0; r < nrep; ++r) {
-        HIPCHK(hipMemcpyAsync(buf.data(), E->st.spins + E->lats[cd.lat0 + r].spin_off, cd.N,
-                              hipMemcpyDeviceToHost, E->stream));
+        const LatDesc &d = E->lats[cd.lat0 + r];
+        HIPCHK(hipMemcpyAsync(d.packed ? bits.data() : buf.data(), E->st.spins + d.spin_off,
+                              d.packed ? cd.N / 8 : cd.N, hipMemcpyDeviceToHost, E->stream));
         HIPCHK(hipStreamSynchronize(E->stream));
+        if (d.packed)
+            for (int64_t i = 0; i < cd.N; ++i) buf[i] = (bits[i >> 3] >> (i & 7)) & 1;
         from_internal<uint8_t, int8_t>(cd.L, cd.H, buf.data(), spins + (int64_t)r * cd.N,
                                        [](uint8_t b) -> int8_t { return b ? 1 : -1; });
     }

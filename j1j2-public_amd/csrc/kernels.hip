@@ -58,18 +58,35 @@ __device__ __forceinline__ UnitGeo unit_geo(int u, int c, int H)
     return g;
 }
 
+// Spins in HBM: one byte b = (s+1)/2 per site in the internal (colour-plane)
+// order, or, for the lattices of the byte-layout resident sweep, one bit per
+// site (bit i & 31 of word i >> 5 for internal index i), which that sweep
+// loads and stores once per cycle.  Dword idx of the byte form (sites
+// 4 idx .. 4 idx + 3) from either: PK = 1 reads nibble idx of the bits.
+template <int PK>
+__device__ __forceinline__ uint32_t rd_word(const uint32_t *sp32, int idx)
+{
+    if constexpr (PK) return (((sp32[idx >> 3] >> (4 * (idx & 7))) & 0xFu) * 0x00204081u) & 0x01010101u;
+    else return sp32[idx];
+}
+
+__device__ __forceinline__ uint32_t spin_at(const uint8_t *sp, int packed, int64_t i)
+{
+    return packed ? (reinterpret_cast<const uint32_t *>(sp)[i >> 5] >> (i & 31)) & 1u : sp[i];
+}
+
 // +x / -x neighbour bytes of dword w of the half-row at dword offset `uoff`,
 // taken from the other-parity half-row of the same row (periodic in x).
-template <int WPH, int HX>
+template <int WPH, int HX, int PK = 0>
 __device__ __forceinline__ void x_pair(const uint32_t *base, int uoff, int w, uint32_t &Xp, uint32_t &Xm)
 {
-    const uint32_t A = base[uoff + w];
+    const uint32_t A = rd_word<PK>(base, uoff + w);
     if constexpr (HX == 0) {
-        const uint32_t B = base[uoff + (w == 0 ? WPH - 1 : w - 1)];
+        const uint32_t B = rd_word<PK>(base, uoff + (w == 0 ? WPH - 1 : w - 1));
         Xp = A;
         Xm = __builtin_amdgcn_alignbyte(A, B, 3);
     } else {
-        const uint32_t B = base[uoff + (w == WPH - 1 ? 0 : w + 1)];
+        const uint32_t B = rd_word<PK>(base, uoff + (w == WPH - 1 ? 0 : w + 1));
         Xm = A;
         Xp = __builtin_amdgcn_alignbyte(B, A, 1);
     }
@@ -78,8 +95,9 @@ __device__ __forceinline__ void x_pair(const uint32_t *base, int uoff, int w, ui
 // Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) over the units of
 // one lattice in the colour-plane layout: l_i = 2 (a(+y) + a(-y) - a(+x) -
 // a(-x)), a = 1 if the bond is anti-aligned.  A site's spin is bit SH of its
-// byte: SH = 0 in global memory, 2 in the resident sweep's LDS copy.
-template <int WPH, int HX, int SH>
+// byte: SH = 0 in global memory, 2 in the resident sweep's LDS copy; PK = 1:
+// global memory holds one bit per site (rd_word).
+template <int WPH, int HX, int SH, int PK = 0>
 __device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
                                              int32_t *locspin, int32_t *locnem)
 {
@@ -88,10 +106,10 @@ __device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H,
         const int u = t / WPH, w = t - u * WPH;
         const UnitGeo g = unit_geo<WPH>(u, c, H);
         const int py = (c ^ 2) * nq, dw = c * nq + t;
-        const uint32_t S = sp32[dw];
-        const uint32_t YP = sp32[py + g.uyp * WPH + w], YM = sp32[py + g.uym * WPH + w];
+        const uint32_t S = rd_word<PK>(sp32, dw);
+        const uint32_t YP = rd_word<PK>(sp32, py + g.uyp * WPH + w), YM = rd_word<PK>(sp32, py + g.uym * WPH + w);
         uint32_t Xp, Xm;
-        x_pair<WPH, HX>(sp32, (c ^ 1) * nq + u * WPH, w, Xp, Xm);
+        x_pair<WPH, HX, PK>(sp32, (c ^ 1) * nq + u * WPH, w, Xp, Xm);
         const uint32_t ax = (S ^ Xp) + (S ^ Xm), ay = (S ^ YP) + (S ^ YM);
         int4 ls = reinterpret_cast<int4 *>(locspin)[dw], lv = reinterpret_cast<int4 *>(locnem)[dw];
         int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
@@ -105,13 +123,13 @@ __device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H,
     }
 }
 
-template <int WPH, int SH>
+template <int WPH, int SH, int PK = 0>
 __device__ __forceinline__ void locnem_units(const uint32_t *sp32, int H, int t0, int stride, int32_t *locspin,
                                              int32_t *locnem)
 {
     for (int c = 0; c < 4; ++c) {
-        if (c & 1) locnem_plane<WPH, 1, SH>(sp32, c, H, t0, stride, locspin, locnem);
-        else locnem_plane<WPH, 0, SH>(sp32, c, H, t0, stride, locspin, locnem);
+        if (c & 1) locnem_plane<WPH, 1, SH, PK>(sp32, c, H, t0, stride, locspin, locnem);
+        else locnem_plane<WPH, 0, SH, PK>(sp32, c, H, t0, stride, locspin, locnem);
     }
 }
 
@@ -486,9 +504,13 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
-    for (int i = tid; i < N / 16; i += NT) {            // bytes b -> 4b in LDS
-        const uint4 v = reinterpret_cast<const uint4 *>(gsp)[i];
-        reinterpret_cast<uint4 *>(lds32)[i] = make_uint4(v.x << 2, v.y << 2, v.z << 2, v.w << 2);
+    for (int i = tid; i < N / 32; i += NT) {            // bits -> bytes 4b in LDS (32 sites per word)
+        const uint32_t W = reinterpret_cast<const uint32_t *>(gsp)[i];
+        uint32_t b[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) b[k] = (((W >> (4 * k)) & 0xFu) * 0x00810204u) & 0x04040404u;
+        reinterpret_cast<uint4 *>(lds32)[2 * i] = make_uint4(b[0], b[1], b[2], b[3]);
+        reinterpret_cast<uint4 *>(lds32)[2 * i + 1] = make_uint4(b[4], b[5], b[6], b[7]);
     }
 
     const ConfDesc cd = st.confs[d.conf];
@@ -565,9 +587,13 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
             atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
         }
     }
-    for (int i = tid; i < N / 16; i += NT) {
-        const uint4 v = reinterpret_cast<const uint4 *>(lds32)[i];
-        reinterpret_cast<uint4 *>(gsp)[i] = make_uint4(v.x >> 2, v.y >> 2, v.z >> 2, v.w >> 2);
+    for (int i = tid; i < N / 32; i += NT) {            // bytes 4b -> bits
+        const uint4 v0 = reinterpret_cast<const uint4 *>(lds32)[2 * i], v1 = reinterpret_cast<const uint4 *>(lds32)[2 * i + 1];
+        const uint32_t b[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        uint32_t W = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) W |= (((((b[k] >> 2) & 0x01010101u) * 0x01020408u) >> 24) & 0xFu) << (4 * k);
+        reinterpret_cast<uint32_t *>(gsp)[i] = W;
     }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
@@ -745,10 +771,10 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
         int x, y, z;
         site_of_internal(L, H, i, x, y, z);
         const Nbr nb = neighbours(L, H, x, y, z);
-        const int si = sp[i] ? 1 : -1;
+        const int si = spin_at(sp, d.packed, i) ? 1 : -1;
         int s[10];
 #pragma unroll
-        for (int t = 0; t < 10; ++t) s[t] = sp[nb.m[t]] ? 1 : -1;
+        for (int t = 0; t < 10; ++t) s[t] = spin_at(sp, d.packed, nb.m[t]) ? 1 : -1;
         v[OBS_M] += si;
         v[OBS_SX] += (x & 1) ? -si : si;
         v[OBS_SY] += (y & 1) ? -si : si;
@@ -904,21 +930,28 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
     const uint32_t *sp32 = reinterpret_cast<const uint32_t *>(sp);
     int32_t *ls = st.locspin + d.site_off, *ln = st.locnem + d.site_off;
     const int t0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
-    switch (L) {
-        case 8: locnem_units<1, 0>(sp32, H, t0, stride, ls, ln); break;
-        case 16: locnem_units<2, 0>(sp32, H, t0, stride, ls, ln); break;
-        case 24: locnem_units<3, 0>(sp32, H, t0, stride, ls, ln); break;
-        case 32: locnem_units<4, 0>(sp32, H, t0, stride, ls, ln); break;
-        case 40: locnem_units<5, 0>(sp32, H, t0, stride, ls, ln); break;
-        case 48: locnem_units<6, 0>(sp32, H, t0, stride, ls, ln); break;
+    const int pk = d.packed;
+    switch (pk ? L : L + 1) {     // packed lattices: L % 8 == 0, L <= 48
+        case 8: locnem_units<1, 0, 1>(sp32, H, t0, stride, ls, ln); break;
+        case 16: locnem_units<2, 0, 1>(sp32, H, t0, stride, ls, ln); break;
+        case 24: locnem_units<3, 0, 1>(sp32, H, t0, stride, ls, ln); break;
+        case 32: locnem_units<4, 0, 1>(sp32, H, t0, stride, ls, ln); break;
+        case 40: locnem_units<5, 0, 1>(sp32, H, t0, stride, ls, ln); break;
+        case 48: locnem_units<6, 0, 1>(sp32, H, t0, stride, ls, ln); break;
+        case 9: locnem_units<1, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 17: locnem_units<2, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 25: locnem_units<3, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 33: locnem_units<4, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 41: locnem_units<5, 0>(sp32, H, t0, stride, ls, ln); break;
+        case 49: locnem_units<6, 0>(sp32, H, t0, stride, ls, ln); break;
         default:
             for (int i = t0; i < N; i += stride) {
                 int x, y, z;
                 site_of_internal(L, H, i, x, y, z);
                 const Nbr nb = neighbours(L, H, x, y, z);
-                const int si = sp[i] ? 1 : -1;
-                const int l = si * ((sp[nb.m[2]] ? 1 : -1) - (sp[nb.m[0]] ? 1 : -1) + (sp[nb.m[3]] ? 1 : -1) -
-                                    (sp[nb.m[1]] ? 1 : -1));
+                const int si = spin_at(sp, pk, i) ? 1 : -1;
+                const int l = si * ((spin_at(sp, pk, nb.m[2]) ? 1 : -1) - (spin_at(sp, pk, nb.m[0]) ? 1 : -1) +
+                                    (spin_at(sp, pk, nb.m[3]) ? 1 : -1) - (spin_at(sp, pk, nb.m[1]) ? 1 : -1));
                 ls[i] += si;
                 ln[i] += l;
             }
