@@ -29,26 +29,26 @@ HBM_PEAK = 8.0e12                  # MI355X HBM3E, bytes/s
 SEED = 20261203
 
 
-def build_batch(J, precision, flags=0):
-    g = J.make_params(L, L, nreplica=NREP, nconf=NCONF, iconf0=1, ptswapever=PTSWAPEVER,
+def build_batch(J, precision, flags=0, nconf=NCONF):
+    g = J.make_params(L, L, nreplica=NREP, nconf=nconf, iconf0=1, ptswapever=PTSWAPEVER,
                       precision=precision, flags=flags | J.FLAG_WRITELOCNEM, tmin=8.0, tmax=11.5,
                       seed=SEED)
     eng = J.Engine([g])
     stripes = J.initial_spins(L, L, J.START_STR, 0, 0)
     import numpy as np
     spins = np.tile(stripes, NREP)
-    for c in range(NCONF):
+    for c in range(nconf):
         eng.upload_disorder(c, J.make_disorder(L, L, 3.0, 1 + c, True, 2.0))
         eng.set_spins(c, spins)
     return eng
 
 
-def time_form(J, precision, steps, warmup, kernel_timing):
+def time_form(J, precision, steps, warmup, kernel_timing, nconf=NCONF):
     """Timed region: `steps` MEAS cycles replayed as hipGraphs.  With
     kernel_timing, a second pass of the same length brackets every sweep
     launch with HIP events on the engine's stream (launched eagerly) for the
     sweep kernel's average duration."""
-    eng = build_batch(J, precision)
+    eng = build_batch(J, precision, nconf=nconf)
     eng.run(J.PHASE_EQUIL, warmup)
     eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
     eng.synchronize()
@@ -66,7 +66,7 @@ def time_form(J, precision, steps, warmup, kernel_timing):
     plan = eng.launch_plan()
     fetch = eng.fetch(0)
     eng.close()
-    su = float(L * L * L) * NREP * NCONF * PTSWAPEVER * steps
+    su = float(L * L * L) * NREP * nconf * PTSWAPEVER * steps
     return dict(seconds=dt, su=su, rate=su / dt, sweep_ms=sweep_ms, launches=launches, path=path,
                 bytes_per=bytes_per, e_slot0=float(fetch["basefields"][9][0]), plan=plan)
 
@@ -298,6 +298,11 @@ def main():
         fp64 = time_form(J, J.FP64_PLAIN, n64, max(args.warmup // 10, 2), kernel_timing=False)
         res["fp64_plain"] = {"value": fp64["rate"], "steps": n64, "ms_per_step": 1e3 * fp64["seconds"] / n64,
                              "fp32_over_fp64": fp32["rate"] / fp64["rate"]}
+        # the same FP64 form run the reference's way, one disorder config at a
+        # time (one engine of 16 replicas)
+        one = time_form(J, J.FP64_PLAIN, 4 * n64, max(args.warmup // 10, 2), kernel_timing=False, nconf=1)
+        res["fp64_plain"]["one_config_at_a_time"] = {"value": one["rate"], "steps": 4 * n64,
+                                                     "fp32_over_fp64": fp32["rate"] / one["rate"]}
         rate, n = mismatch_rate(J, 4)
         res["flip_mismatch_rate"] = {"fp32_vs_fp64": rate, "proposals": n}
     if not args.no_cpu:

@@ -188,7 +188,7 @@ int ref_site(int L, int x, int y, int z) { return x + L * y + L * L * z; }
 // host-side FP32 constants, fixed-point disorder and FP64 field of one config,
 // written in the internal site order
 void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p, std::vector<float> kd[5],
-                       std::vector<double> &p64, std::vector<int32_t> &qh, std::vector<int32_t> &ql)
+                       std::vector<double> &p64, std::vector<uint8_t> &qb, int64_t &qsum)
 {
     const int L = cd.L, H = cd.H, N = cd.N;
     std::vector<float> Kx(N), Ky(N);
@@ -201,8 +201,8 @@ void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p,
             }
     for (int P = 0; P < 5; ++P) kd[P].assign(N, 0.0f);
     p64.assign(N, 0.0);
-    qh.assign(N, 0);
-    ql.assign(N, 0);
+    qb.assign((size_t)5 * N, 0);
+    qsum = 0;
     for (int z = 0; z < H; ++z)
         for (int y = 0; y < L; ++y)
             for (int x = 0; x < L; ++x) {
@@ -218,9 +218,10 @@ void build_site_planes(const ConfDesc &cd, const ConfConst &cc, const double *p,
                 kd[3][ii] = std::ldexp(2.0f * kyp, 14);
                 kd[4][ii] = std::ldexp(2.0f * kym, 14);
                 p64[ii] = p[i];
-                const int64_t q = (int64_t)std::llrint(p[i] * 0x1p32);     // |q| < 2^38
-                qh[ii] = (int32_t)(q >> 16);
-                ql[ii] = (int32_t)(q & 0xffff);
+                const int64_t q = (int64_t)std::llrint(p[i] * 0x1p32);     // |q| < 2^38 (|p| < 64)
+                const uint64_t qo = (uint64_t)(q + QOFF);                   // five byte planes of q + 2^38
+                for (int b = 0; b < 5; ++b) qb[(size_t)b * N + ii] = (uint8_t)(qo >> (8 * b));
+                qsum += q;
             }
 }
 
@@ -229,10 +230,9 @@ int upload_site_planes(Engine &E, int conf, const double *p)
     const ConfDesc &cd = E.confs[conf];
     std::vector<float> kd[5];
     std::vector<double> p64;
-    std::vector<int32_t> qh, ql;
-    build_site_planes(cd, E.cconst[conf], p, kd, p64, qh, ql);
+    std::vector<uint8_t> qb;
     int64_t qsum = 0;
-    for (int i = 0; i < cd.N; ++i) qsum += (int64_t)qh[i] * 65536 + ql[i];
+    build_site_planes(cd, E.cconst[conf], p, kd, p64, qb, qsum);
     E.cconst[conf].qsum = qsum;
     HIPCHK(hipMemcpyAsync(const_cast<ConfConst *>(E.st.cconst) + conf, &E.cconst[conf], sizeof(ConfConst),
                           hipMemcpyHostToDevice, E.stream));
@@ -241,10 +241,9 @@ int upload_site_planes(Engine &E, int conf, const double *p)
                               sizeof(float) * cd.N, hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipMemcpyAsync(const_cast<double *>(E.st.p64) + cd.site_off, p64.data(), sizeof(double) * cd.N,
                           hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipMemcpyAsync(const_cast<int32_t *>(E.st.qh) + cd.site_off, qh.data(), sizeof(int32_t) * cd.N,
-                          hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipMemcpyAsync(const_cast<int32_t *>(E.st.ql) + cd.site_off, ql.data(), sizeof(int32_t) * cd.N,
-                          hipMemcpyHostToDevice, E.stream));
+    for (int b = 0; b < 5; ++b)
+        HIPCHK(hipMemcpyAsync(const_cast<uint8_t *>(E.st.qb) + b * E.nsites + cd.site_off, qb.data() + (size_t)b * cd.N,
+                              cd.N, hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
     return 0;
 }
@@ -937,10 +936,10 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     DeviceState &st = E->st;
     const int64_t nconf = (int64_t)E->confs.size();
     LatDesc *lats; ConfDesc *confs; ConfConst *cconst;
-    float *kd, *b2; double *p64, *beta; int32_t *q;
+    float *kd, *b2; double *p64, *beta; uint8_t *q;
     if ((rc = E->alloc(lats, nlat)) || (rc = E->alloc(confs, nconf)) || (rc = E->alloc(cconst, nconf)) ||
         (rc = E->alloc(st.spins, E->spin_bytes)) || (rc = E->alloc(kd, 5 * E->nsites)) ||
-        (rc = E->alloc(p64, E->nsites)) || (rc = E->alloc(q, 2 * E->nsites)) ||
+        (rc = E->alloc(p64, E->nsites)) || (rc = E->alloc(q, 5 * E->nsites)) ||
         (rc = E->alloc(st.obs, nlat * OBS_N)) || (rc = E->alloc(st.rep_at_temp, E->nslots)) ||
         (rc = E->alloc(st.temp_of_rep, E->nslots)) || (rc = E->alloc(beta, E->nslots)) ||
         (rc = E->alloc(b2, E->nslots)) || (rc = E->alloc(st.clk, 1)) ||
@@ -966,7 +965,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         return rc;
     st.lats = lats; st.confs = confs; st.cconst = cconst;
     st.nlat = (int32_t)nlat; st.nconf = (int32_t)nconf; st.nsites = E->nsites;
-    st.kd = kd; st.p64 = p64; st.qh = q; st.ql = q + E->nsites; st.beta = beta; st.b2 = b2;
+    st.kd = kd; st.p64 = p64; st.qb = q; st.beta = beta; st.b2 = b2;
     st.series_cap = series_cap;
     st.ptswapever = c0.ptswapever; st.ptmeasindex = c0.ptmeasindex; st.bins = c0.bins;
     st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;

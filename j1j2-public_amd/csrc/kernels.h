@@ -16,6 +16,10 @@ struct ConfConst {
     int64_t qsum;              // sum over the sites of llrint(p * 2^32)
 };
 
+// Offset that makes the fixed-point disorder q = llrint(p * 2^32), |q| < 2^38,
+// non-negative for its five byte planes.
+constexpr int64_t QOFF = int64_t(1) << 38;
+
 struct DeviceState {
     // geometry
     const LatDesc *lats;       // nlat
@@ -28,7 +32,7 @@ struct DeviceState {
     uint8_t *spins;            // sum over lattices of N bytes, internal layout
     const float *kd;           // 5 planes x nsites: kc, then 2^14 * (2Kx+, 2Kx-, 2Ky+, 2Ky-)
     const double *p64;         // nsites
-    const int32_t *qh, *ql;    // nsites each: llrint(p * 2^32) = qh * 2^16 + ql, 0 <= ql < 2^16
+    const uint8_t *qb;         // 5 planes x nsites: byte b of llrint(p * 2^32) + QOFF (|p| < 64)
     int64_t *obs;              // nlat x OBS_N
     // ladder
     int32_t *rep_at_temp;      // nslots (per config blocks of nrep)

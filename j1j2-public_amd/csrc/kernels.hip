@@ -134,20 +134,18 @@ __device__ __forceinline__ void locnem_units(const uint32_t *sp32, int H, int t0
 }
 
 // Per-thread partial sums of the integer observables in int32 (a thread covers
-// at most a few hundred sites), except the high part of the field sum.
+// at most a few hundred sites) and the field term's sum q e (QSum).
 struct ObsAcc {
-    int32_t m, sx, sy, bx, by, bd, bz, eql;
-    int64_t eqh;
+    int32_t m, sx, sy, bx, by, bd, bz;
+    QSum q;
 };
 
 // Integer observables (see k_observables) of one colour plane of the resident
 // sweep's LDS copy (spin = bit 2 of its byte), forward bonds, one dword
-// (4 sites) per step.  The field term sum q_i l_i is split as q = qh 2^16 + ql
-// (|qh| < 2^22, 0 <= ql < 2^16, |l| <= 4) so that every product is a 24-bit
-// multiply: sum qh l per dword fits int32 and is widened, sum ql l stays int32.
+// (4 sites) per step; the field term as sum q e (QSum).
 template <int WPH, int HX>
 __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
-                                              const int32_t *qh, const int32_t *ql, ObsAcc &a)
+                                              const QPlanes &qp, ObsAcc &a)
 {
     const int nq = 4 * WPH * WPH * H;
     for (int t = t0; t < nq; t += stride) {
@@ -169,17 +167,8 @@ __device__ __forceinline__ void observe_plane(const uint32_t *sp32, int c, int H
         a.bz += 4 - 2 * __builtin_popcount(S ^ ZP);
         a.bd += 8 - 2 * (__builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1));
         const uint32_t ax = (S ^ Xp0) + (S ^ Xm0), ay = (S ^ YP) + (S ^ YM);   // bytes 0, 4, 8
-        const int dw = c * nq + t;
-        const int4 h4 = reinterpret_cast<const int4 *>(qh)[dw], l4 = reinterpret_cast<const int4 *>(ql)[dw];
-        const int hq[4] = {h4.x, h4.y, h4.z, h4.w}, lq[4] = {l4.x, l4.y, l4.z, l4.w};
-        int32_t sh = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int l = 2 * ((int)((ay >> (8 * j + 2)) & 0xffu) - (int)((ax >> (8 * j + 2)) & 0xffu));
-            sh += __mul24(hq[j], l);
-            a.eql += __mul24(lq[j], l);
-        }
-        a.eqh += sh;
+        const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;                     // bytes l/2 + 2
+        qsum_word(a.q, qp, (uint32_t)(c * nq + t) * 4u, e);
     }
 }
 
@@ -250,8 +239,8 @@ constexpr int fast_zs()
 }
 
 template <int WPH, int NT, int HX>
-__device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const int32_t *qh,
-                                             const int32_t *ql, ObsFast &a)
+__device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const QPlanes &qp,
+                                             ObsFast &a)
 {
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = fast_zs<WPH, NT>();
     const int nq = PER_Z * H, tid = threadIdx.x;
@@ -269,8 +258,6 @@ __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H
     const int dY = ((c ^ 2) - c) * nq;
     const int own0 = c * nq + z0 * PER_Z + tz;
     uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0, ps1 = 0, nw1 = 0;
-    int64_t eh = 0;
-    int32_t el = 0;
 #pragma unroll 2
     for (int z = z0; z < H; z += ZSTEP) {
         const bool alt = (ZSTEP & 1) && (((z - z0) / ZSTEP) & 1);
@@ -296,21 +283,12 @@ __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H
         pd += __builtin_popcount(S ^ Xp1) + __builtin_popcount(S ^ Xm1);
         const uint32_t ax = sx0 + (S ^ Xm0), ay = sy0 + (S ^ YM);          // bytes 0, 4, 8
         const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;                 // bytes l/2 + 2
-        const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
-        const int4 h4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(qh) + ob);
-        const int4 l4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(ql) + ob);
-        const int32_t sh = __mul24(h4.x, (int)(e & 0xffu)) + __mul24(h4.y, (int)((e >> 8) & 0xffu)) +
-                           __mul24(h4.z, (int)((e >> 16) & 0xffu)) + __mul24(h4.w, (int)(e >> 24));
-        el += __mul24(l4.x, (int)(e & 0xffu)) + __mul24(l4.y, (int)((e >> 8) & 0xffu)) +
-              __mul24(l4.z, (int)((e >> 16) & 0xffu)) + __mul24(l4.w, (int)(e >> 24));
-        eh += sh;
+        qsum_word(a.q, qp, (uint32_t)own * 4u, e);
     }
     const int32_t m = 2 * (int32_t)ps - 4 * (int32_t)nw, m1 = 2 * (int32_t)ps1 - 4 * (int32_t)nw1;
     a.ps += ps + ps1; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += nw + nw1;
     a.sxm += HX ? -(m + m1) : m + m1;
     a.sym += (g.p ? -m : m) + (g1.p ? -m1 : m1);
-    a.eh += eh;
-    a.el += el;
 }
 
 // Hide how an LDS address was formed from the optimiser: it stays one
@@ -555,11 +533,12 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 
     // ---- observables of the final state (integer sums; phi part fixed point)
     int64_t v[OBS_N];
+    const QPlanes qp = qplanes(st.qb + d.site_off, st.nsites);
     if constexpr (fast_zs<WPH, NT>() >= 2) {
-        ObsFast a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        ObsFast a = {};
         for (int c = 0; c < 4; ++c) {
-            if (c & 1) observe_fast<WPH, NT, 1>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
-            else observe_fast<WPH, NT, 0>(lds32, c, H, st.qh + d.site_off, st.ql + d.site_off, a);
+            if (c & 1) observe_fast<WPH, NT, 1>(lds32, c, H, qp, a);
+            else observe_fast<WPH, NT, 0>(lds32, c, H, qp, a);
         }
         const int32_t nw = (int32_t)a.nw;
         v[OBS_M] = 2 * (int64_t)a.ps - 4 * (int64_t)nw;
@@ -569,15 +548,15 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         v[OBS_BY] = 4 * (int64_t)nw - 2 * (int64_t)a.py;
         v[OBS_BD] = 8 * (int64_t)nw - 2 * (int64_t)a.pd;
         v[OBS_BZ] = 4 * (int64_t)nw - 2 * (int64_t)a.pz;
-        v[OBS_EQ] = 2 * (a.eh * 65536 + a.el);        // the -4 sum q term is added after the reduction
+        v[OBS_EQ] = 2 * qsum_total(a.q);             // the -4 sum q term is added after the reduction
     } else {
-        ObsAcc a = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        ObsAcc a = {};
         for (int c = 0; c < 4; ++c) {
-            if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
-            else observe_plane<WPH, 0>(lds32, c, H, tid, NT, st.qh + d.site_off, st.ql + d.site_off, a);
+            if (c & 1) observe_plane<WPH, 1>(lds32, c, H, tid, NT, qp, a);
+            else observe_plane<WPH, 0>(lds32, c, H, tid, NT, qp, a);
         }
         v[OBS_M] = a.m; v[OBS_SX] = a.sx; v[OBS_SY] = a.sy; v[OBS_BX] = a.bx; v[OBS_BY] = a.by;
-        v[OBS_BD] = a.bd; v[OBS_BZ] = a.bz; v[OBS_EQ] = a.eqh * 65536 + a.eql;
+        v[OBS_BD] = a.bd; v[OBS_BZ] = a.bz; v[OBS_EQ] = 2 * qsum_total(a.q);
     }
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
@@ -597,7 +576,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
-        if constexpr (fast_zs<WPH, NT>() >= 2) v[OBS_EQ] -= 4 * cc.qsum;
+        v[OBS_EQ] -= 4 * cc.qsum;
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
         for (int k = 0; k < OBS_N; ++k) o[k] = v[k];
@@ -765,7 +744,7 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
     const LatDesc d = st.lats[lat];
     const int L = d.L, H = d.H, N = d.N;
     const uint8_t *sp = st.spins + d.spin_off;
-    const int32_t *qh = st.qh + d.site_off, *ql = st.ql + d.site_off;
+    const uint8_t *qb = st.qb + d.site_off;
     int64_t v[OBS_N] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < N; i += OBS_THREADS) {
         int x, y, z;
@@ -782,7 +761,10 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
         v[OBS_BY] += si * s[0];
         v[OBS_BD] += si * s[4] + si * s[5];
         v[OBS_BZ] += si * s[8];
-        v[OBS_EQ] += ((int64_t)qh[i] * 65536 + ql[i]) * (int64_t)(si * (s[2] + s[3] - s[0] - s[1]));
+        int64_t q = -QOFF;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) q += (int64_t)qb[b * st.nsites + i] << (8 * b);
+        v[OBS_EQ] += q * (int64_t)(si * (s[2] + s[3] - s[0] - s[1]));
     }
     block_reduce_obs<OBS_THREADS>(v, red);
     if (threadIdx.x == 0) {

@@ -238,8 +238,8 @@ __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, co
 // kernels.hip, one 4-site group per step; +z bonds, so a slab reads its upper
 // halo).
 template <int L, int HX, bool SLAB>
-__device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H, const Share &sh, const int32_t *qh,
-                                            const int32_t *ql, ObsFast &a)
+__device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H, const Share &sh,
+                                            const QPlanes &qp, ObsFast &a)
 {
     using Gm = BitGeo<L>;
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
@@ -248,8 +248,7 @@ __device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H,
     const int yh = tid / G, j = tid - yh * G, m0 = 4 * j;
     const int nq = HW * H * G;
     uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0;
-    int32_t sxm = 0, sym = 0, el = 0;
-    int64_t eh = 0;
+    int32_t sxm = 0, sym = 0;
     for (int zl = 0; zl < sh.nz; ++zl) {
         const int zg = sh.z0 + zl;
         const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
@@ -268,16 +267,10 @@ __device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H,
         sym += n.p ? -m : m;
         const uint32_t ax = sx0 + (S ^ s.xm), ay = sy0 + (S ^ s.ym);   // bytes 0, 4, 8
         const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;             // bytes l/2 + 2
-        const uint32_t ob = (uint32_t)(c * nq + zg * TPL + tid) * 16u;
-        const int4 h4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(qh) + ob);
-        const int4 l4 = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(ql) + ob);
-        eh += __mul24(h4.x, (int)(e & 0xffu)) + __mul24(h4.y, (int)((e >> 8) & 0xffu)) +
-              __mul24(h4.z, (int)((e >> 16) & 0xffu)) + __mul24(h4.w, (int)(e >> 24));
-        el += __mul24(l4.x, (int)(e & 0xffu)) + __mul24(l4.y, (int)((e >> 8) & 0xffu)) +
-              __mul24(l4.z, (int)((e >> 16) & 0xffu)) + __mul24(l4.w, (int)(e >> 24));
+        qsum_word(a.q, qp, (uint32_t)(c * nq + zg * TPL + tid) * 4u, e);
     }
     a.ps += ps; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += nw;
-    a.sxm += sxm; a.sym += sym; a.eh += eh; a.el += el;
+    a.sxm += sxm; a.sym += sym;
 }
 
 // Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) from the bit layout
@@ -486,10 +479,11 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
         __syncthreads();
     }
 
-    ObsFast a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    ObsFast a = {};
+    const QPlanes qp = qplanes(st.qb + d.site_off, st.nsites);
     for (int c = 0; c < 4; ++c) {
-        if (c & 1) bit_observe<L, 1, SLAB>(lds32, c, H, sh, st.qh + d.site_off, st.ql + d.site_off, a);
-        else bit_observe<L, 0, SLAB>(lds32, c, H, sh, st.qh + d.site_off, st.ql + d.site_off, a);
+        if (c & 1) bit_observe<L, 1, SLAB>(lds32, c, H, sh, qp, a);
+        else bit_observe<L, 0, SLAB>(lds32, c, H, sh, qp, a);
     }
     const int64_t nw = (int64_t)a.nw;
     int64_t v[OBS_N];
@@ -500,7 +494,7 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     v[OBS_BY] = 4 * nw - 2 * (int64_t)a.py;
     v[OBS_BD] = 8 * nw - 2 * (int64_t)a.pd;
     v[OBS_BZ] = 4 * nw - 2 * (int64_t)a.pz;
-    v[OBS_EQ] = 2 * (a.eh * 65536 + a.el);
+    v[OBS_EQ] = 2 * qsum_total(a.q);
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
         sh_mis = wave_sum<unsigned long long>(sh_mis);

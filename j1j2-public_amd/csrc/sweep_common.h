@@ -112,14 +112,58 @@ __device__ __forceinline__ float f16_site(uint32_t even, uint32_t odd, int j)
 }
 
 
+// The field term of the energy, sum_i q_i l_i with q_i = llrint(p_i 2^32)
+// (|q_i| < 2^38) and l_i = 2 (a(+y) + a(-y) - a(+x) - a(-x)) in -4..4 (a = 1
+// for an anti-aligned bond), is summed as 2 sum q_i e_i - 4 sum q_i with
+// e_i = l_i / 2 + 2 in 0..4, the last sum a per-config constant.  The
+// disorder is held as five byte planes of q_i + 2^38, so for a word of four
+// sites (e_i in its bytes) one v_dot4_u32_u8 per plane adds the four products
+// e_i * byte_b(q_i + 2^38) exactly into a 32-bit partial sum (at most 1020
+// per site), and a sixth adds sum e_i for the offset QOFF (kernels.h).
+
+struct QSum {
+    uint32_t a[5], se;
+};
+
+// The config's five byte planes (block-uniform pointers).
+struct QPlanes {
+    const uint8_t *p[5];
+};
+
+__device__ __forceinline__ QPlanes qplanes(const uint8_t *qb, int64_t nsites)
+{
+    QPlanes q;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) q.p[b] = qb + b * nsites;
+    return q;
+}
+
+// boff: 4 x the word's dword index within the config's planes (a 32-bit
+// offset from the uniform plane bases)
+__device__ __forceinline__ void qsum_word(QSum &s, const QPlanes &q, uint32_t boff, uint32_t e)
+{
+#pragma unroll
+    for (int b = 0; b < 5; ++b)
+        s.a[b] = __builtin_amdgcn_udot4(e, *reinterpret_cast<const uint32_t *>(q.p[b] + boff), s.a[b], false);
+    s.se = __builtin_amdgcn_udot4(e, 0x01010101u, s.se, false);
+}
+
+// sum q_i e_i over the words added
+__device__ __forceinline__ int64_t qsum_total(const QSum &s)
+{
+    int64_t t = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) t += (int64_t)s.a[b] << (8 * b);
+    return t - QOFF * (int64_t)s.se;
+}
+
 // Per-thread partial observables of the fixed-geometry paths: raw popcounts
-// and the field term as sum q e (see observe_fast).
+// and the field term's sum q e (see observe_fast).
 struct ObsFast {
     uint32_t nw;                       // words visited
     uint32_t ps, px, py, pz, pd;       // popcounts of S, S^X+, S^Y+, S^Z+, S^D(+x+y) + S^D(-x+y)
     int32_t sxm, sym;                  // signed sums of 2 popc(S) - 4 (x and y stripe)
-    int64_t eh;                        // sum qh e
-    int32_t el;                        // sum ql e
+    QSum q;                            // sum q e
 };
 
 }  // namespace
