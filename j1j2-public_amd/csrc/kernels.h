@@ -29,7 +29,7 @@ struct DeviceState {
     int64_t nsites;            // sum over configs of N (per-site planes)
     const int32_t *order;      // resident launches: lattice of each block (-1: none), per launch segment
     // lattice state
-    uint8_t *spins;            // sum over lattices of N bytes, internal layout
+    uint8_t *spins;            // per lattice (LatDesc::spin_off): N bytes, or N/8 when packed; internal order
     const float *kd;           // 5 planes x nsites: kc, then 2^14 * (2Kx+, 2Kx-, 2Ky+, 2Ky-)
     const double *p64;         // nsites
     const uint8_t *qb;         // 5 planes x nsites: byte b of llrint(p * 2^32) + QOFF (|p| < 64)
