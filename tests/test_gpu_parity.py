@@ -68,6 +68,20 @@ def test_resident_block_sizes_match_oracle(nt, monkeypatch):
         eng.close()
 
 
+@pytest.mark.parametrize("L,H,tmin,tmax", [(16, 8, 0.02, 0.05), (16, 8, 2.0e4, 1.0e5), (64, 4, 0.02, 0.05),
+                                          (64, 4, 2.0e4, 1.0e5)])
+def test_fp32_extreme_temperatures_match_oracle(L, H, tmin, tmax):
+    """Far outside the benchmark's temperatures the hardware-exp2 filter of
+    the decision meets its edge cases: 2^y below the normal range (every
+    uphill move rejected unless r = 0) and y near 0 (thresholds within 2^-16
+    of 2^32, left open and decided by the exact rule); byte and bit layouts."""
+    say(f"case fp32 extreme T L={L} H={H} T={tmin}..{tmax}")
+    g = J.make_params(L, H, nreplica=3, nconf=2, iconf0=11, precision=J.FP32_BATCHED, flags=LOCNEM,
+                      seed=SEED, ptmeasindex=2, tmin=tmin, tmax=tmax)
+    eng, n = run_and_compare([g], "hot", 3, 4)
+    assert eng.sweep_path()[0] in (1, 3) and n == 2
+
+
 @pytest.mark.parametrize("L,H,R,start", [(56, 4, 3, "hot"), (64, 2, 2, "str"), (72, 4, 2, "hot"),
                                           (80, 2, 3, "hot"), (88, 4, 2, "str"), (96, 4, 3, "hot")])
 def test_fp32_resident_bits_matches_oracle(L, H, R, start):
