@@ -796,6 +796,25 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     const uint64_t cycle = st.clk->cycle;
     const int phase = ((cycle - st.clk->phase_start) & 1ull) ? 2 : 1;
     const int64_t slen = st.local ? st.series_len[conf] : 0;
+    // The kernel is a chain of dependent memory round trips; everything that
+    // does not depend on this cycle's energies is requested up front: the
+    // betas of this thread's first swap pair and its first slot's running
+    // sums, and counters are bumped with atomics that return nothing.
+    const int start = (phase == 1) ? 0 : 1;
+    const int s0 = start + 2 * tid;
+    const bool pair0 = !st.local && swaps && s0 < nrep - 1;
+    const double b_lo = pair0 ? st.beta[sb + s0] : 0.0, b_hi = pair0 ? st.beta[sb + s0 + 1] : 0.0;
+    const bool slot0 = measure && !st.local && tid < nrep;
+    double S0[NSUMS], best0 = 0.0, hmin0 = 0.0, hmax0 = 0.0, hbin0 = 0.0;
+    if (slot0) {
+        const double *S = st.sums + (int64_t)(sb + tid) * NSUMS;
+#pragma unroll
+        for (int q = 0; q < NSUMS; ++q) S0[q] = S[q];
+        best0 = st.best[sb + tid];
+        hmin0 = st.hmin[sb + tid];
+        hmax0 = st.hmax[sb + tid];
+        hbin0 = st.hbin[sb + tid];
+    }
     for (int r = tid; r < nrep; r += PT_THREADS) {
         const int64_t *o = st.obs + (int64_t)(cd.lat0 + r) * OBS_N;
         const double ej = ((cc.J1h * (double)o[OBS_BX]) + (cc.J1v * (double)o[OBS_BY])) +
@@ -814,16 +833,16 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     }
     __syncthreads();
     if (!st.local && swaps) {
-        const int start = (phase == 1) ? 0 : 1;
-        for (int s = start + 2 * tid; s < nrep - 1; s += 2 * PT_THREADS) {
+        for (int s = s0; s < nrep - 1; s += 2 * PT_THREADS) {
             const int rlo = rat[s], rhi = rat[s + 1];
-            st.att[cd.pair_base + s] += 1;
-            const double delta = (st.beta[sb + s] - st.beta[sb + s + 1]) * (et[rlo] - et[rhi]);
+            atomicAdd(reinterpret_cast<unsigned long long *>(st.att + cd.pair_base + s), 1ull);
+            const double blo = s == s0 ? b_lo : st.beta[sb + s], bhi = s == s0 ? b_hi : st.beta[sb + s + 1];
+            const double delta = (blo - bhi) * (et[rlo] - et[rhi]);
             const U4 rn = philox4x32_10((uint32_t)s, 0x80000000u, (uint32_t)cycle, (uint32_t)(cycle >> 32),
                                         st.seed, (uint32_t)cd.iconf);
             const double u = (double)rn.x * 0x1p-32;
             if (delta >= 0.0 || u < exp64(delta)) {
-                st.acc[cd.pair_base + s] += 1;
+                atomicAdd(reinterpret_cast<unsigned long long *>(st.acc + cd.pair_base + s), 1ull);
                 rat[s] = rhi;
                 rat[s + 1] = rlo;
                 tor[rlo] = s + 1;
@@ -851,31 +870,47 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
                 }
                 continue;
             }
+            double Sv[NSUMS], best, hmin, hmax, hbin;
+            double *S = st.sums + (int64_t)(sb + sl) * NSUMS;
+            if (sl == tid) {
+#pragma unroll
+                for (int q = 0; q < NSUMS; ++q) Sv[q] = S0[q];
+                best = best0; hmin = hmin0; hmax = hmax0; hbin = hbin0;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NSUMS; ++q) Sv[q] = S[q];
+                best = st.best[sb + sl]; hmin = st.hmin[sb + sl]; hmax = st.hmax[sb + sl]; hbin = st.hbin[sb + sl];
+            }
             const double ax2 = ax * ax, ay2 = ay * ay, m2 = am * am, e2 = ae * ae, n2 = an * an;
             const double str2 = ax2 + ay2;
-            double *S = st.sums + (int64_t)(sb + sl) * NSUMS;
-            S[0] += am;      S[1] += fabs(am);      S[2] += m2;         S[3] += m2 * m2;
-            S[4] += ae;      S[5] += e2;            S[6] += fabs(ax);   S[7] += fabs(ay);
-            S[8] += ax2;     S[9] += ax2 * ax2;     S[10] += ay2;       S[11] += ay2 * ay2;
-            S[12] += str2;   S[13] += str2 * str2;  S[14] += sqrt(str2); S[15] += fabs(ay2 - ax2);
-            S[16] += an;     S[17] += fabs(an);     S[18] += n2;        S[19] += n2 * n2;
-            if (ae < st.best[sb + sl]) st.best[sb + sl] = ae;
+            Sv[0] += am;      Sv[1] += fabs(am);      Sv[2] += m2;          Sv[3] += m2 * m2;
+            Sv[4] += ae;      Sv[5] += e2;            Sv[6] += fabs(ax);    Sv[7] += fabs(ay);
+            Sv[8] += ax2;     Sv[9] += ax2 * ax2;     Sv[10] += ay2;        Sv[11] += ay2 * ay2;
+            Sv[12] += str2;   Sv[13] += str2 * str2;  Sv[14] += sqrt(str2); Sv[15] += fabs(ay2 - ax2);
+            Sv[16] += an;     Sv[17] += fabs(an);     Sv[18] += n2;         Sv[19] += n2 * n2;
+#pragma unroll
+            for (int q = 0; q < NSUMS; ++q) S[q] = Sv[q];
+            if (ae < best) st.best[sb + sl] = ae;
             st.final_en[sb + sl] = ae;
-            if (st.hbin[sb + sl] <= 0.0) {
-                st.hmin[sb + sl] = ae - (fabs(ae) + 1.0);
-                st.hmax[sb + sl] = ae + (fabs(ae) + 1.0);
-                st.hbin[sb + sl] = (st.hmax[sb + sl] - st.hmin[sb + sl]) / (double)st.bins;
+            if (hbin <= 0.0) {
+                hmin = ae - (fabs(ae) + 1.0);
+                hmax = ae + (fabs(ae) + 1.0);
+                hbin = (hmax - hmin) / (double)st.bins;
+                st.hmin[sb + sl] = hmin;
+                st.hmax[sb + sl] = hmax;
+                st.hbin[sb + sl] = hbin;
             }
-            const double tq = (ae - st.hmin[sb + sl]) / st.hbin[sb + sl];
+            const double tq = (ae - hmin) / hbin;
             int idx;
             if (tq < 0.0) idx = 1;
             else if (tq >= (double)st.bins) idx = st.bins;
             else idx = 1 + (int)tq;
             idx = idx < 1 ? 1 : (idx > st.bins ? st.bins : idx);
-            st.hist[cd.hist_off + (int64_t)sl * st.bins + idx - 1] += 1;
+            atomicAdd(reinterpret_cast<unsigned long long *>(st.hist + cd.hist_off + (int64_t)sl * st.bins + idx - 1),
+                      1ull);
         }
         if (tid == 0) {
-            st.nmeasure[conf] += 1;
+            atomicAdd(reinterpret_cast<unsigned long long *>(st.nmeasure + conf), 1ull);
             if (st.local) st.series_len[conf] = slen + 1;
         }
     }
