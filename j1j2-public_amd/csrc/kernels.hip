@@ -916,16 +916,16 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     }
     if (swaps) {
         if (tid == 0 && st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
-        // the last block to finish advances the cycle clock (every block has
-        // read it before incrementing `done`)
+        // the last block to finish advances the cycle clock: every thread of
+        // every block has used its read of the clock before its block's
+        // barrier and increment of `done`, and the next kernel sees the new
+        // value across the launch boundary, so no fence is needed
         __syncthreads();
         if (tid == 0) {
-            __threadfence();
             const unsigned prev = atomicAdd(&st.clk->done, 1u);
             if (prev == gridDim.x - 1) {
                 st.clk->done = 0;
                 st.clk->cycle = cycle + 1;
-                __threadfence();
             }
         }
     }
