@@ -664,7 +664,10 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         maxc = std::max(maxc, cand[i].size());
     }
     const bool multi = R.size() > 1;
-    const int need = TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_byte ? 3 : 0)) : 0);
+    static constexpr int32_t ALL_NT[] = {256, 512, 576, 1024};     // one launch over all lattices
+    constexpr int n_all = (int)(sizeof(ALL_NT) / sizeof(ALL_NT[0]));
+    // cycles the candidates below may take (an upper bound: the call must hold them all)
+    const int need = TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_byte ? n_all : 0)) : 0);
     if (ncycles - done < need) return 0;             // too short a call; tune on a later one
     int rc;
     // 1. threads per block of each group, its launch timed alone
@@ -702,9 +705,11 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         std::vector<Plan> alt;
         Plan conc = per;
         conc.concurrent = true;
-        alt.push_back(conc);
+        // slab launches need all their blocks resident at once: not beside others
+        if (std::none_of(per.launches.begin(), per.launches.end(), [](const ResLaunch &r) { return r.slabs > 1; }))
+            alt.push_back(conc);
         if (E.all_byte)
-            for (int32_t nt : {256, 512, 576, 1024}) {
+            for (int32_t nt : ALL_NT) {
                 Plan one;
                 one.launches.push_back(all_launch(E, nt));
                 alt.push_back(one);
