@@ -24,9 +24,9 @@ namespace {
 // resident at once): a slab holds its own layers plus one halo layer below and
 // above per plane; after colour pass c it publishes its two boundary layers of
 // plane c as exchange words in global memory, each carrying a version number,
-// and before pass c' it waits for its neighbours' words of plane c'^2 (the
-// only plane whose z-neighbours pass c' reads) with the version of that
-// plane's last update.  A slot is never overwritten before it is read: a slab
+// and during the pass before c' it waits for its neighbours' words of plane
+// c'^2 (the only plane whose z-neighbours pass c' reads) with the version of
+// that plane's last update.  A slot is never overwritten before it is read: a slab
 // republishes plane c only after a pass that waits for the reader's later
 // publication.  64-bit agent-scope atomic loads and stores carry each word
 // whole, so no fences are needed.
@@ -456,12 +456,18 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
             if constexpr (SLAB) {
-                // plane c^2 was last updated in pass c^2 of this sweep (c >= 2) or
-                // of the previous one (c < 2; at k = 0 the halos just loaded)
-                if (c >= 2 || k > 0) {
-                    const uint32_t ver = vbase + 4u * (uint32_t)(c >= 2 ? k : k - 1) + (uint32_t)(c ^ 2) + 1u;
-                    slab_fetch<HW>(st, lds32, sh, lat, s, S, c ^ 2, ver & XV_MASK);
-                    __syncthreads();
+                // The halos of the next pass (kn, cn) are fetched during this one:
+                // it reads plane cn^2 only in its halo layers, which this pass
+                // neither reads (its z-neighbours are in plane c^2) nor writes,
+                // and the barrier after it orders the writes before pass cn.
+                // Plane cn^2 was last updated in pass cn^2 of sweep kn (cn >= 2)
+                // or kn - 1 (cn < 2; at kn = 0 the halos loaded with the slab),
+                // published by the neighbours before they could need this
+                // pass's words, so the wait always ends.
+                const int cn = (c + 1) & 3, kn = c == 3 ? k + 1 : k;
+                if (kn < st.ptswapever && (cn >= 2 || kn > 0)) {
+                    const uint32_t ver = vbase + 4u * (uint32_t)(cn >= 2 ? kn : kn - 1) + (uint32_t)(cn ^ 2) + 1u;
+                    slab_fetch<HW>(st, lds32, sh, lat, s, S, cn ^ 2, ver & XV_MASK);
                 }
             }
             if (c & 1) bit_pass<L, 1, SHADOW, SLAB>(lds32, cx, cc, c, H, sh, sh_prop, sh_mis);
