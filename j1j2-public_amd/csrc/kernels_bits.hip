@@ -13,12 +13,14 @@ namespace j1j2 {
 namespace {
 
 // LDS layout of one lattice: plane c (colour), unit u = z*HW + y/2 (a half-row
-// of HW = L/2 same-colour sites), one 64-bit word per unit at dwords
-// 2*(c*NU + u), NU = HW*H units per plane.  Bit 1 + m holds site m (m = x/2)
-// of the half-row, bit 0 repeats site HW-1 and bit HW+1 repeats site 0, so the
-// four sites m0+s .. m0+s+3 (m0 = 4j, s in {-1, 0, +1}) are always bits
-// m0+1+s .. m0+4+s of the word, wrap included.  One spare dword follows the
-// last word (the two-dword reads of the last unit touch it).
+// of HW = L/2 same-colour sites), two dwords per unit at dwords 2*(c*NU + u),
+// NU = HW*H units per plane.  Dword 0 holds sites 0..N0-1 (m = x/2) at bits
+// 1..N0, dword 1 sites N0..HW-1 at bits 1..N1 (N0 = 4 ceil(G/2) and N1 = HW-N0
+// sites, G = HW/4 groups); bit 0 of each dword repeats the site before its
+// first one (dword 0: site HW-1, dword 1: site N0-1) and bit N+1 the site
+// after its last one (dword 0: site N0, dword 1: site 0).  So the four sites
+// of a group and the same window shifted by one site either way (its x
+// neighbours, wrap included) are one 4-bit field of one dword.
 //
 // A lattice may instead be split into S z-slabs, one block each (all of them
 // resident at once): a slab holds its own layers plus one halo layer below and
@@ -36,13 +38,33 @@ struct BitGeo {
     static constexpr int G = HW / 4;                 // 4-site groups per half-row (= Philox groups)
     static constexpr int TPL = G * HW;               // groups per z-layer of a plane: one thread each
     static constexpr int NT = (TPL + 63) / 64 * 64;  // threads per block (idle tail lanes)
+    static constexpr int G0 = (G + 1) / 2;           // groups in dword 0 of a unit
+    static constexpr int N0 = 4 * G0, N1 = HW - N0;  // sites in dwords 0 and 1 (<= 24 each)
 };
 
-// Four sites starting at bit `pos` of the unit word at dword `dw`.
+// The two dwords of a unit from its sites (bit m of v = site m, m < HW).
+template <int L>
+__device__ __forceinline__ uint2 unit_from_sites(unsigned long long v)
+{
+    constexpr int N0 = BitGeo<L>::N0, N1 = BitGeo<L>::N1;
+    const uint32_t lo = (uint32_t)v & ((1u << N0) - 1u), hi = (uint32_t)(v >> N0) & ((1u << N1) - 1u);
+    return make_uint2((lo << 1) | (hi >> (N1 - 1)) | ((hi & 1u) << (N0 + 1)),
+                      (hi << 1) | (lo >> (N0 - 1)) | ((lo & 1u) << (N1 + 1)));
+}
+
+// The sites of a unit (bit m = site m) from its two dwords.
+template <int L>
+__device__ __forceinline__ unsigned long long sites_from_unit(uint2 d)
+{
+    constexpr int N0 = BitGeo<L>::N0, N1 = BitGeo<L>::N1;
+    return (unsigned long long)((d.x >> 1) & ((1u << N0) - 1u)) |
+           ((unsigned long long)((d.y >> 1) & ((1u << N1) - 1u)) << N0);
+}
+
+// Four sites starting at bit `pos` of LDS dword `dw`.
 __device__ __forceinline__ uint32_t nib(const uint32_t *lds32, int dw, int pos)
 {
-    const int d = dw + (pos >> 5);
-    return __builtin_amdgcn_alignbit(lds32[d + 1], lds32[d], (uint32_t)(pos & 31)) & 0xFu;
+    return __builtin_amdgcn_ubfe(lds32[dw], (uint32_t)pos, 4u);
 }
 
 // Site k of a nibble into bit 8k+2 (the byte layout of the LDS copy in
@@ -88,17 +110,30 @@ struct BitSites {
     uint32_t d1p, d1m, d2p, d2m;      // diagonals of rows y+1 (d1) and y-1 (d2)
 };
 
+// Where group j of a unit sits: dword h of the unit, sites at bits pos..pos+3.
+struct GroupPos {
+    int h, pos;
+};
+
+template <int L>
+__device__ __forceinline__ GroupPos group_pos(int j)
+{
+    constexpr int G0 = BitGeo<L>::G0;
+    const int h = j >= G0 ? 1 : 0;
+    return GroupPos{h, 4 * (j - h * G0) + 1};
+}
+
 // HX: x parity of the plane's sites; for even sites +x is the other plane's
 // site m and -x its site m-1, for odd sites -x is m and +x is m+1.  NU: units
-// per plane in LDS.
+// per plane in LDS; gp: the group's place in its units.
 template <int HW, int HX>
-__device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int NU, const BitNb &n, int m0)
+__device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int NU, const BitNb &n, GroupPos gp)
 {
-    const int pos0 = m0 + 1, posx = m0 + 1 + (HX ? 1 : -1);
-    const int p1 = 2 * (c ^ 1) * NU, p2 = 2 * (c ^ 2) * NU, p3 = 2 * (c ^ 3) * NU;
+    const int pos0 = gp.pos, posx = gp.pos + (HX ? 1 : -1);
+    const int p1 = 2 * (c ^ 1) * NU + gp.h, p2 = 2 * (c ^ 2) * NU + gp.h, p3 = 2 * (c ^ 3) * NU + gp.h;
     const int u = n.u;
     BitSites s;
-    s.S = nib(lds32, 2 * (c * NU + u), pos0);
+    s.S = nib(lds32, 2 * (c * NU + u) + gp.h, pos0);
     const uint32_t x0 = spread4(nib(lds32, p1 + 2 * u, pos0)), x1 = spread4(nib(lds32, p1 + 2 * u, posx));
     s.xp = HX ? x1 : x0;
     s.xm = HX ? x0 : x1;
@@ -138,9 +173,9 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x, m0 = 4 * j;
     const int NUg = HW * H, nq = NUg * G;
-    unsigned long long *lds64 = reinterpret_cast<unsigned long long *>(lds32);
     const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
-    const BitSites s = bit_sites<HW, HX>(lds32, c, NU, n, m0);
+    const GroupPos gp = group_pos<L>(j);
+    const BitSites s = bit_sites<HW, HX>(lds32, c, NU, n, gp);
     const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
     const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
     const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
@@ -204,10 +239,13 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
         for (int k = 0; k < 4; ++k)
             if (((open >> k) & 1u) && decide_f32_open(ys[k], ra[k])) f |= 1u << k;
     }
-    unsigned long long mask = (unsigned long long)f << (m0 + 1);
-    if (j == 0) mask |= (unsigned long long)(f & 1u) << (HW + 1);     // copy of site 0
-    if (j == G - 1) mask |= (unsigned long long)(f >> 3);             // copy of site HW-1
-    atomicXor(&lds64[c * NU + n.u], mask);
+    // flips into the group's dword; the first and last group of each dword
+    // also flip their site's copy in the other dword of the unit
+    constexpr int G0 = Gm::G0, N0 = Gm::N0, N1 = Gm::N1;
+    uint32_t *uw = lds32 + 2 * (c * NU + n.u);
+    atomicXor(uw + gp.h, f << gp.pos);
+    if (j == 0 || j == G0) atomicXor(uw + (1 - gp.h), (f & 1u) << (j == 0 ? N1 + 1 : N0 + 1));
+    if (j == G0 - 1 || j == G - 1) atomicXor(uw + (1 - gp.h), f >> 3);
 }
 
 // One colour pass over the block's layers.  Whole lattice: two layers per
@@ -245,14 +283,14 @@ __device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H,
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
-    const int yh = tid / G, j = tid - yh * G, m0 = 4 * j;
+    const int yh = tid / G, j = tid - yh * G;
     const int nq = HW * H * G;
     uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0;
     int32_t sxm = 0, sym = 0;
     for (int zl = 0; zl < sh.nz; ++zl) {
         const int zg = sh.z0 + zl;
         const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, m0);
+        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, group_pos<L>(j));
         const uint32_t S = spread4(s.S);
         const uint32_t sx0 = S ^ s.xp, sy0 = S ^ s.yp;
         const int pcs = __builtin_popcount(s.S);
@@ -283,12 +321,12 @@ __device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, 
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
-    const int yh = tid / G, j = tid - yh * G, m0 = 4 * j;
+    const int yh = tid / G, j = tid - yh * G;
     const int nq = HW * H * G;
     for (int zl = 0; zl < sh.nz; ++zl) {
         const int zg = sh.z0 + zl;
         const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, m0);
+        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, group_pos<L>(j));
         const uint32_t S = spread4(s.S);
         const uint32_t ax = (S ^ s.xp) + (S ^ s.xm), ay = (S ^ s.yp) + (S ^ s.ym);
         const int dw = c * nq + zg * TPL + tid;
@@ -305,7 +343,7 @@ __device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, 
 }
 
 // ---- slab exchange words
-constexpr int XV_SHIFT = 50;                             // unit words use bits 0 .. HW+1 <= 49
+constexpr int XV_SHIFT = 50;                             // a unit's HW <= 48 sites below, the version above
 constexpr unsigned long long XW_MASK = (1ull << XV_SHIFT) - 1;
 constexpr uint32_t XV_MASK = (1u << (64 - XV_SHIFT)) - 1;
 constexpr int SPIN_MAX = 1 << 21;                        // polls before a wait gives up (seconds)
@@ -348,31 +386,33 @@ __device__ __forceinline__ unsigned long long *xword(const DeviceState &st, int 
 }
 
 // After pass c: the slab's boundary layers of plane c, version ver.
-template <int HW>
+template <int L>
 __device__ __forceinline__ void slab_publish(const DeviceState &st, const uint32_t *lds32, const Share &sh, int lat,
                                              int s, int c, uint32_t ver)
 {
+    constexpr int HW = BitGeo<L>::HW;
     const int tid = threadIdx.x;
     if (tid < 2 * HW) {
         const int w = tid >= HW, yh = tid - w * HW;
         const unsigned long long v =
-            reinterpret_cast<const unsigned long long *>(lds32)[c * sh.NU + (w ? sh.nz : 1) * HW + yh];
-        xstore(xword<HW>(st, lat, s, c, w, yh), (v & XW_MASK) | ((unsigned long long)ver << XV_SHIFT));
+            sites_from_unit<L>(reinterpret_cast<const uint2 *>(lds32)[c * sh.NU + (w ? sh.nz : 1) * HW + yh]);
+        xstore(xword<HW>(st, lat, s, c, w, yh), v | ((unsigned long long)ver << XV_SHIFT));
     }
 }
 
 // The halo layers of plane c from the slabs below (its upper layer) and above
 // (its lower layer), version ver; the caller synchronises before reading them.
-template <int HW>
+template <int L>
 __device__ __forceinline__ void slab_fetch(const DeviceState &st, uint32_t *lds32, const Share &sh, int lat, int s,
                                            int S, int c, uint32_t ver)
 {
+    constexpr int HW = BitGeo<L>::HW;
     const int tid = threadIdx.x;
     if (tid < 2 * HW) {
         const int w = tid >= HW, yh = tid - w * HW;            // w = 0: lower halo, 1: upper
         const int src = w ? (s + 1 == S ? 0 : s + 1) : (s == 0 ? S - 1 : s - 1);
         const unsigned long long x = xwait(xword<HW>(st, lat, src, c, 1 - w, yh), XV_SHIFT, ver, st.slab_err);
-        reinterpret_cast<unsigned long long *>(lds32)[c * sh.NU + (w ? sh.nz + 1 : 0) * HW + yh] = x & XW_MASK;
+        reinterpret_cast<uint2 *>(lds32)[c * sh.NU + (w ? sh.nz + 1 : 0) * HW + yh] = unit_from_sites<L>(x & XW_MASK);
     }
 }
 
@@ -400,15 +440,13 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
             zg = zg < 0 ? zg + H : (zg >= H ? zg - H : zg);
         }
         const uint8_t *src = gsp + ((int64_t)c * NUg + (int64_t)zg * HW + yh) * HW;
-        unsigned long long w = 0;
+        unsigned long long v = 0;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
             const uint32_t b = *reinterpret_cast<const uint32_t *>(src + 4 * k);
-            w |= (unsigned long long)((b * 0x10204080u) >> 28) << (1 + 4 * k);   // bytes 0..3 -> bits
+            v |= (unsigned long long)((b * 0x10204080u) >> 28) << (4 * k);   // bytes 0..3 -> bits
         }
-        w |= (w >> HW) & 1ull;                                 // site HW-1 at bit 0
-        w |= (w & 2ull) << HW;                                 // site 0 at bit HW+1
-        reinterpret_cast<unsigned long long *>(lds32)[U] = w;
+        reinterpret_cast<uint2 *>(lds32)[U] = unit_from_sites<L>(v);
     }
     if (tid == 0) lds32[8 * sh.NU] = 0;                         // the spare dword
 
@@ -467,21 +505,21 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
                 const int cn = (c + 1) & 3, kn = c == 3 ? k + 1 : k;
                 if (kn < st.ptswapever && (cn >= 2 || kn > 0)) {
                     const uint32_t ver = vbase + 4u * (uint32_t)(cn >= 2 ? kn : kn - 1) + (uint32_t)(cn ^ 2) + 1u;
-                    slab_fetch<HW>(st, lds32, sh, lat, s, S, cn ^ 2, ver & XV_MASK);
+                    slab_fetch<L>(st, lds32, sh, lat, s, S, cn ^ 2, ver & XV_MASK);
                 }
             }
             if (c & 1) bit_pass<L, 1, SHADOW, SLAB>(lds32, cx, cc, c, H, sh, sh_prop, sh_mis);
             else bit_pass<L, 0, SHADOW, SLAB>(lds32, cx, cc, c, H, sh, sh_prop, sh_mis);
             __syncthreads();
-            if constexpr (SLAB) slab_publish<HW>(st, lds32, sh, lat, s, c, (vbase + 4u * k + c + 1u) & XV_MASK);
+            if constexpr (SLAB) slab_publish<L>(st, lds32, sh, lat, s, c, (vbase + 4u * k + c + 1u) & XV_MASK);
         }
     }
     if constexpr (SLAB) {
         // the final planes 2 and 3 of the neighbours (planes 0 and 1 were
         // fetched after their last update); the observables read the upper halo
         const uint32_t kl = (uint32_t)st.ptswapever - 1u;
-        slab_fetch<HW>(st, lds32, sh, lat, s, S, 2, (vbase + 4u * kl + 3u) & XV_MASK);
-        slab_fetch<HW>(st, lds32, sh, lat, s, S, 3, (vbase + 4u * kl + 4u) & XV_MASK);
+        slab_fetch<L>(st, lds32, sh, lat, s, S, 2, (vbase + 4u * kl + 3u) & XV_MASK);
+        slab_fetch<L>(st, lds32, sh, lat, s, S, 3, (vbase + 4u * kl + 4u) & XV_MASK);
         __syncthreads();
     }
 
@@ -513,11 +551,11 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     const int off = SLAB ? HW : 0;
     for (int U = tid; U < 4 * sh.nz * HW; U += NT) {
         const int c = U / (sh.nz * HW), r = U - c * sh.nz * HW;
-        const unsigned long long w = reinterpret_cast<const unsigned long long *>(lds32)[c * sh.NU + off + r];
+        const unsigned long long v = sites_from_unit<L>(reinterpret_cast<const uint2 *>(lds32)[c * sh.NU + off + r]);
         uint8_t *dst = gsp + ((int64_t)c * NUg + (int64_t)sh.z0 * HW + r) * HW;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
-            const uint32_t n4 = (uint32_t)(w >> (1 + 4 * k)) & 0xFu;
+            const uint32_t n4 = (uint32_t)(v >> (4 * k)) & 0xFu;
             *reinterpret_cast<uint32_t *>(dst + 4 * k) = (n4 * 0x00204081u) & 0x01010101u;
         }
     }
