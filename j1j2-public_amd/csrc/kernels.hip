@@ -482,13 +482,26 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
-    for (int i = tid; i < N / 32; i += NT) {            // bits -> bytes 4b in LDS (32 sites per word)
-        const uint32_t W = reinterpret_cast<const uint32_t *>(gsp)[i];
-        uint32_t b[8];
+    // bits -> bytes 4b in LDS (32 sites per word); a thread's words are all
+    // requested before the first is unpacked (one memory round trip, not four)
+    constexpr int LDU = 4;
+    for (int i0 = tid; i0 < N / 32; i0 += LDU * NT) {
+        uint32_t W[LDU];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) b[k] = (((W >> (4 * k)) & 0xFu) * 0x00810204u) & 0x04040404u;
-        reinterpret_cast<uint4 *>(lds32)[2 * i] = make_uint4(b[0], b[1], b[2], b[3]);
-        reinterpret_cast<uint4 *>(lds32)[2 * i + 1] = make_uint4(b[4], b[5], b[6], b[7]);
+        for (int q = 0; q < LDU; ++q) {
+            const int i = i0 + q * NT;
+            W[q] = i < N / 32 ? reinterpret_cast<const uint32_t *>(gsp)[i] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < LDU; ++q) {
+            const int i = i0 + q * NT;
+            if (i >= N / 32) break;
+            uint32_t b[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) b[k] = (((W[q] >> (4 * k)) & 0xFu) * 0x00810204u) & 0x04040404u;
+            reinterpret_cast<uint4 *>(lds32)[2 * i] = make_uint4(b[0], b[1], b[2], b[3]);
+            reinterpret_cast<uint4 *>(lds32)[2 * i + 1] = make_uint4(b[4], b[5], b[6], b[7]);
+        }
     }
 
     const ConfDesc cd = st.confs[d.conf];
