@@ -244,8 +244,11 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     constexpr int G0 = Gm::G0, N0 = Gm::N0, N1 = Gm::N1;
     uint32_t *uw = lds32 + 2 * (c * NU + n.u);
     atomicXor(uw + gp.h, f << gp.pos);
-    if (j == 0 || j == G0) atomicXor(uw + (1 - gp.h), (f & 1u) << (j == 0 ? N1 + 1 : N0 + 1));
-    if (j == G0 - 1 || j == G - 1) atomicXor(uw + (1 - gp.h), f >> 3);
+    // (masks fixed per thread, so one conditional LDS op and no branch tree)
+    const uint32_t fm = (j == 0 || j == G0) ? 1u : 0u, lm = (j == G0 - 1 || j == G - 1) ? 1u : 0u;
+    const uint32_t fsh = j == 0 ? N1 + 1 : N0 + 1;
+    const uint32_t cm = ((f & fm) << fsh) | ((f >> 3) & lm);
+    if (cm) atomicXor(uw + (1 - gp.h), cm);
 }
 
 // One colour pass over the block's layers.  Whole lattice: two layers per
