@@ -182,14 +182,16 @@ int  j1j2_get_mismatch(void *handle, int32_t conf, int64_t *proposals, int64_t *
  * 2 = observables, 3 = local nematic.  Returns total ms and launch count. */
 int  j1j2_set_timing(void *handle, int32_t enable);
 int  j1j2_get_timing(void *handle, int32_t kind, double *total_ms, int64_t *launches);
-/* Which sweep path the engine picked (0 = FP64 plain, 1 = FP32 LDS-resident,
- * 2 = FP32 global) and the algorithmic bytes per site update it is priced at. */
+/* Which sweep path the engine picked (0 = FP64 plain, 1 = FP32 LDS-resident
+ * byte layout for every lattice, 2 = FP32 global memory for at least one
+ * lattice, 3 = FP32 LDS-resident with the bit layout for some lattices) and the
+ * algorithmic bytes per site update it is priced at. */
 int  j1j2_sweep_path(void *handle, int32_t *path, double *bytes_per_update);
 /* The resident sweep's launch plan as text: per launch the lattice size, the
  * threads per block and the lattice count, whether the launches run on
  * parallel graph branches, and whether the autotuner picked it.  The engine
  * tunes on the first call of j1j2_run / j1j2_run_cycles long enough for the
- * candidates (at most 16 cycles, counted in the call's cycles); the result of
+ * candidates (at most 18 cycles, counted in the call's cycles); the result of
  * a run does not depend on the plan.  No counterpart in the reference. */
 int  j1j2_launch_plan(void *handle, char *buf, int32_t len);
 int  j1j2_synchronize(void *handle);
