@@ -80,3 +80,21 @@ def test_large_l96(flags, path, slabs, monkeypatch):
     ora = oracle_for(gg, iconf, p, s)
     ora.run(1, False)
     compare_state(eng, 3, ora, J.ladder(gg), measured=False)
+
+
+@pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
+def test_maximum_lattice(precision):
+    """The largest lattice the engine takes (L*L*H = 2^22: L = 128, H = 256, on
+    the global-memory sweep), one equilibration and one measurement cycle, bit
+    for bit against the oracle."""
+    say(f"case maximum lattice 128x128x256 precision {precision}")
+    g = J.make_params(128, 256, nreplica=2, nconf=1, iconf0=1, precision=precision, seed=SEED)
+    eng, inputs = build_engine([g], "hot")
+    assert eng.sweep_path()[0] == (2 if precision == J.FP32_BATCHED else 0)
+    eng.run(J.PHASE_EQUIL, 1)
+    eng.run(J.PHASE_MEAS, 1)
+    gg, iconf, p, s = inputs[0]
+    ora = oracle_for(gg, iconf, p, s)
+    ora.run(1, False)
+    ora.run(1, True)
+    compare_state(eng, 0, ora, J.ladder(gg))

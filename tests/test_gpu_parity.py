@@ -206,6 +206,24 @@ def test_flip_counts_match_oracle():
     run_and_compare([g], "hot", 2, 3, flips=True)
 
 
+@pytest.mark.parametrize("precision,flags,R,ptswapever,ptmeasindex,bins", [
+    (J.FP32_BATCHED, 0, 1, 3, 1, 7),                  # one temperature: no swap pairs
+    (J.FP32_BATCHED, 0, 5, 3, 5, 13),                 # PTMEASINDEX at the last slot, odd PTSWAPEVER
+    (J.FP32_BATCHED, J.FLAG_FORCE_GLOBAL, 2, 2, 2, 1),  # one histogram bin, global-memory sweep
+    (J.FP64_PLAIN, 0, 1, 2, 1, 100),
+    (J.FP64_PLAIN, 0, 4, 1, 4, 3)])
+def test_pt_edge_parameters_match_oracle(precision, flags, R, ptswapever, ptmeasindex, bins):
+    """Run parameters at the edges of their ranges (J1J2_6tv.f90:841-925 accepts
+    NREPLICA = 1, PTSWAPEVER = 1, PTMEASINDEX = NREPLICA, BINS = 1)."""
+    say(f"case PT edges precision {precision} flags {flags} R={R} swapever {ptswapever} "
+        f"measindex {ptmeasindex} bins {bins}")
+    g = J.make_params(8, 4, nreplica=R, nconf=2, iconf0=5, precision=precision, ptswapever=ptswapever,
+                      ptmeasindex=ptmeasindex, bins=bins, flags=flags | LOCNEM | J.FLAG_WRITENUMFLIPS,
+                      seed=SEED)
+    eng, n = run_and_compare([g], ["hot", "str"], 2, 4, flips=True)
+    assert n == 2
+
+
 def test_zero_cycles_report_initial_state(golden_dir):
     """NEQ = NMESS = 0: the engine reports the HOT starts' energies exactly as
     the reference's cfen table does (fixture from the reference build)."""
