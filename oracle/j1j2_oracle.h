@@ -15,19 +15,23 @@
  * oracle is what the GPU results must equal bit for bit; agreement with the
  * reference itself is statistical and is checked separately.
  *
- * Pins (see the tests that exercise them):
- *   kiss         -> outputs of the reference's own rkiss05/kissinit built from
- *                   its sources (tests/golden/kiss_*.txt)
+ * Pins (tests/test_oracle.py exercises each):
+ *   kiss, corr field -> the reference's own phi_sites.dat for three (L, H,
+ *                   seed), pow-2 and padded grids (tests/golden/ref_*), which
+ *                   only the exact KISS seeding and stream, polar Gaussian,
+ *                   filter, normalisation and erf map reproduce
+ *   initial spins -> the reference's cfen of HOT and STR starts run with
+ *                   NEQ = NMESS = 0
  *   philox       -> published Random123 known-answer vectors
- *   exp2/exp64   -> libm exp2/exp in long double over dense grids
+ *   exp64, threshold32 -> exp / 2^y in double over dense grids
  *   fields       -> brute-force dE = E(after) - E(before); closed forms of
  *                   the stripe and ferro states
  *   sweep rule   -> beta = 0 flips every site; T -> 0 accepts only dE <= 0;
  *                   exact enumeration of 2x2x4 equilibrium averages
  *   swap rule    -> equal temperatures accept every attempt
  *   derived      -> closed forms on synthetic series
- *   corr field   -> outputs of the reference's make_corr_rand_3d built from
- *                   its sources (tests/golden/corr_*.txt) and textbook DFT
+ *   LOCAL histogram -> the reference's own LOCAL cfhs outputs (a dynamic
+ *                   run occupies bins 25..75, a frozen run only bin 50)
  */
 #ifndef J1J2_ORACLE_H
 #define J1J2_ORACLE_H

@@ -7,7 +7,9 @@
 set -euo pipefail
 HERE=$(cd "$(dirname "$0")" && pwd)
 REPO=$(cd "$HERE/../.." && pwd)
-variants=(hot_init_l8 hot_init_l6h4 str_init_l12 pt_l8_stat local_l8)
+# usage: make_golden.sh [variant ...]     (default: all variants)
+variants=(hot_init_l8 hot_init_l6h4 str_init_l12 pt_l8_stat local_l8 local_frozen)
+[ $# -gt 0 ] && variants=("$@")
 "$REPO/oracle/ref_build.sh" "${variants[@]}"
 for v in "${variants[@]}"; do
     tmp=$(mktemp -d /tmp/j1j2gold.XXXXXX)

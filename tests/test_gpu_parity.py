@@ -237,6 +237,35 @@ def test_zero_cycles_report_initial_state(golden_dir):
     assert np.allclose(tab[:, 1], en.mean(0), rtol=0, atol=1e-9)
 
 
+@pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
+def test_frozen_local_run_matches_reference(golden_dir, precision):
+    """LOCAL, FER start, phi = 0, T = 0.01, NEQ = 10, NMESS = 200: no move is
+    ever taken (dE = 4 at every site), so the engine's energy histogram and
+    energy equal the reference's own run of these parameters (fixture
+    ref_local_frozen) exactly: every sample in bin 50 of the degenerate
+    range e -+ 1e-6."""
+    import os
+    from refdata import find, read_table
+    say(f"case frozen LOCAL run vs reference, precision {precision}")
+    L = 8
+    g = J.make_params(L, L, nreplica=1, ptswapever=1, nconf=1, sampler=J.SAMPLER_LOCAL, precision=precision,
+                      ladder=J.LADDER_LOCAL, tmin=0.01, tmax=0.01, dt=1.0, neq=10, nmess=200, seed=SEED)
+    eng = J.Engine([g])
+    eng.upload_disorder(0, np.zeros(L * L * L))
+    eng.set_spins(0, J.initial_spins(L, L, J.START_FER, 0, 0))
+    eng.run(J.PHASE_EQUIL, 10)
+    eng.run(J.PHASE_MEAS, 200)
+    out = eng.fetch(0)
+    ref_dir = os.path.join(golden_dir, "ref_local_frozen")
+    hs = read_table(find(ref_dir, "cfhs"))
+    assert np.array_equal(out["hist"][:99, 0], hs[:, 3])
+    assert out["hist_min"][0] == -1.0 - 1.0e-6
+    # bin energies as Write_files prints them (e12.6: half an ulp is 5e-6 at |e| = 1)
+    assert np.allclose(out["hist_min"][0] + out["hist_bin"][0] * (hs[:, 1] - 1), hs[:, 2], rtol=0, atol=5e-6)
+    en = read_table(find(ref_dir, "cfen"))
+    assert out["basefields"][9, 0] == en[0, 1] == -1.0
+
+
 def test_pt_ladder_continues_across_calls():
     """run_cycles without begin continues the swap alternation and the sums."""
     say("case chunked runs")

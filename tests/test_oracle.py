@@ -237,6 +237,45 @@ def test_derived_closed_forms():
     assert o2[3] == 0.0 and abs(o2[5] - (1 - 1 / 3)) < 1e-15
 
 
+def _occupied_bins(hist):
+    nz = np.nonzero(hist)[0] + 1                           # 1-based bin numbers
+    return nz.min(), nz.max()
+
+
+def test_local_histogram_rule_matches_reference(golden_dir):
+    """Process_data's histogram (J1J2_6tv.f90:2296-2350) from the reference's
+    own LOCAL outputs.  A dynamic run (cfhs of ref_local_l8): the range is the
+    series' [min, max] widened by half its width on both sides and a sample
+    goes to bin NINT((e - lo) / width), so the lowest energy lands in bin 25
+    and the highest in bin 75 whatever the series, and no sample falls
+    outside (frequencies sum to 1).  A frozen run (ref_local_frozen: FER
+    start, phi = 0, T = 0.01, every energy -1): the range is e -+ 1e-6 and
+    all samples land in bin 50.  A floor+1 binning, another widening or a
+    dropped degenerate branch moves these bins."""
+    # rows: T, bin, energy, frequency (Write_files prints bins 1..BINS-1)
+    tab = read_table(find(os.path.join(golden_dir, "ref_local_l8"), "cfhs"))
+    assert np.array_equal(tab[:, 1], np.arange(1, 100))
+    assert _occupied_bins(tab[:, 3]) == (25, 75) and abs(tab[:, 3].sum() - 1.0) < 1e-5
+    rng = np.random.default_rng(5)
+    n = 1000
+    series = [rng.uniform(-1, 1, n), rng.normal(-4.5, 0.4, n), rng.uniform(-1, 1, n),
+              rng.uniform(-1, 1, n), rng.uniform(-4, 4, n)]
+    _, h, hmin, hbin = O.process_local(*series, 0.1, 512, 100)
+    assert _occupied_bins(h) == (25, 75) and abs(h.sum() - 1.0) < 1e-12
+    e = series[1]
+    assert hmin == e.min() - 0.5 * (e.max() - e.min()) and hbin == 2.0 * (e.max() - e.min()) / 100
+
+    frozen = read_table(find(os.path.join(golden_dir, "ref_local_frozen"), "cfhs"))
+    en = read_table(find(os.path.join(golden_dir, "ref_local_frozen"), "cfen"))
+    assert en[0, 1] == -1.0                                # FER closed form: no move was taken
+    assert np.array_equal(frozen[:, 1], np.arange(1, 100))
+    assert _occupied_bins(frozen[:, 3]) == (50, 50) and frozen[49, 3] == 1.0
+    const = [np.full(200, v) for v in (1.0, -1.0, 0.0, 0.0, 0.0)]
+    _, h, hmin, hbin = O.process_local(*const, 100.0, 512, 100)
+    assert _occupied_bins(h) == (50, 50) and h[49] == 1.0
+    assert hmin == -1.0 - 1.0e-6 and hbin == ((-1.0 + 1.0e-6) - (-1.0 - 1.0e-6)) / 100
+
+
 def test_ladders():
     t = O.ladder(0, 16, 8.0, 11.5)
     assert t[0] == 8.0 and abs(t[-1] - 11.5) < 1e-15 and np.allclose(np.diff(t), 3.5 / 15)
