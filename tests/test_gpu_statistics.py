@@ -10,6 +10,10 @@
 3. Both forms against exact enumeration of the 2x2x4 lattice (2^16 states):
    64 independent LOCAL chains at one temperature, z = (mean of the chains'
    time averages - exact) / (std / sqrt(64)).
+4. Against the reference's own production runs at L = 20 and 40 (phi = 0,
+   across the stripe transition): 100 chains per temperature on both sides,
+   z = (reference - mean) / sqrt(err_ref^2 + err^2) with the standard errors
+   over the chains.
 
 Every z-score must satisfy |z| < 4 and at least 75% (resp. 85%) of them |z| < 2.
 """
@@ -174,3 +178,45 @@ def test_engine_matches_exact_enumeration_2x2x4(precision):
         say(f"  {name:5s} engine {mu:.6f} exact {r:.6f} z {zz:.2f}")
     zsummary(z, f"enumeration_2x2x4_{'fp32' if precision == J.FP32_BATCHED else 'fp64'}")
     assert np.all(np.abs(z) < 4.0)
+
+
+@pytest.mark.parametrize("L,nsweep", [(20, 100000), (40, 50000)])
+def test_local_matches_reference_production_clean(golden_dir, L, nsweep):
+    """The reference's own production runs (data/raw/corr_random_field/p0/c3/
+    J2-1_P0.00_RAN_STRX_s100_scanclean_no_field/L20 and L40, copied to
+    tests/golden/ref_prod_clean with their run_meta.txt): LOCAL sampler, phi = 0,
+    STR-X start, T = 4.00..4.40 in steps of 0.04 with a fresh start at each
+    temperature (INDEP), 100 chains (configs), NEQ = NMESS = 100000 sweeps; cfst
+    gives per temperature <|str|>, the stripe susceptibility and the stripe
+    Binder cumulant, each with its standard error over the 100 chains.  The
+    engine runs the same: one chain per temperature and config, 100 configs of
+    the zero field (L = 20 on the global-memory sweep, L = 40 LDS-resident),
+    nsweep + nsweep sweeps; z = (reference - mean) / sqrt(err_ref^2 + err^2)
+    across the stripe transition (T_c ~ 4.16-4.2)."""
+    say(f"case LOCAL L={L} phi=0 vs the reference's production run ({nsweep}+{nsweep} sweeps)")
+    NT, C = 11, 100
+    g = J.make_params(L, L, nreplica=NT, ptswapever=1, nconf=C, sampler=J.SAMPLER_LOCAL, ladder=J.LADDER_LOCAL,
+                      tmin=4.0, tmax=4.4, dt=0.04, neq=nsweep, nmess=nsweep, seed=31337)
+    eng = J.Engine([g])
+    start = np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), NT)
+    for c in range(C):
+        eng.upload_disorder(c, np.zeros(L * L * L))
+        eng.set_spins(c, start)
+    assert eng.sweep_path()[0] == (2 if L % 8 else 1)
+    eng.run(J.PHASE_EQUIL, nsweep)
+    eng.run(J.PHASE_MEAS, nsweep)
+    bf = np.stack([eng.fetch(c)["basefields"] for c in range(C)])          # config, field, temperature
+    eng.close()
+    tab = read_table(find(os.path.join(golden_dir, "ref_prod_clean"), f"cfst{L:04d}{L:04d}"))
+    assert np.allclose(tab[:, 0], J.ladder(g), atol=5e-6)
+    zs = []
+    # (name, BASEFIELDS index, cfst column, cfst column of its standard error)
+    for name, f, col, ecol in [("|str|", 13, 1, 9), ("chi_str", 18, 6, 10), ("U_str", 20, 7, 8)]:
+        ref, err_ref = tab[:, col], tab[:, ecol]
+        mu, err = bf[:, f, :].mean(0), bf[:, f, :].std(0) / np.sqrt(C)
+        z = (ref - mu) / np.sqrt(err_ref ** 2 + err ** 2)
+        zs.extend(z.tolist())
+        say(f"  {name:8s} ref {np.array2string(ref, precision=4)}\n           engine "
+            f"{np.array2string(mu, precision=4)}\n           z {np.array2string(z, precision=2)}")
+    frac = zsummary(zs, f"local_l{L}_clean_vs_reference_production")
+    assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
