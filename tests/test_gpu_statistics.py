@@ -220,3 +220,58 @@ def test_local_matches_reference_production_clean(golden_dir, L, nsweep):
             f"{np.array2string(mu, precision=4)}\n           z {np.array2string(z, precision=2)}")
     frac = zsummary(zs, f"local_l{L}_clean_vs_reference_production")
     assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
+
+
+# (name, BASEFIELDS index, table, column, column of its standard error over configs)
+PROD_QUANTITIES = [("e", 9, "cfen", 1, 2), ("c", 11, "cfen", 3, 4), ("chi", 3, "cfma", 4, 5),
+                   ("U_m", 5, "cfma", 6, 7), ("|str|", 13, "cfst", 1, 9), ("chi_str", 18, "cfst", 6, 10),
+                   ("U_str", 20, "cfst", 7, 8), ("nbar", 37, "cfnm", 1, 2), ("U_n", 41, "cfnm", 6, 7)]
+
+
+LONG = pytest.mark.skipif(not os.environ.get("J1J2_LONG_TESTS"),
+                          reason="long run (~90 s on one MI355X); set J1J2_LONG_TESTS=1")
+
+
+@pytest.mark.parametrize("L,neq,nmess", [(20, 2000, 4000), (24, 2000, 4000),
+                                         pytest.param(40, 100000, 100000, marks=LONG)])
+def test_pt_matches_reference_production(golden_dir, L, neq, nmess):
+    """The reference's own production PT runs (data/raw/corr_random_field/p2/c3/
+    J2-1_P0.00_RAN_STRX_s100_scanpt_prod_main/L20, L24, L40, copied to
+    tests/golden/ref_prod_pt with their run_meta.txt): PT SERIAL, phi = 2,
+    xi = 3, 20 replicas over T = 8.0..11.5, STR-X start, PTSWAPEVER = 5,
+    NEQ = NMESS = 1e5 cycles, disorder averages over 1000 configs (L = 40: 100)
+    with their standard errors.  The engine runs 64 other disorder configs
+    (iconf 1001..1064: independent draws of the same distribution, seeds the
+    reference's generator would give them); z = (reference - mean) /
+    sqrt(err_ref^2 + err^2) for energy, specific heat, magnetic and stripe
+    susceptibilities and Binder cumulants, <|str|>, nbar and the nematic
+    Binder cumulant at every temperature.  (<|m|> is left out: the reference
+    reports |time average of m|, which shrinks with the run length.)  L = 20
+    and 24 equilibrate within 2000 + 4000 cycles; at L = 40 the swap
+    acceptance is ~0.2% at the cold end and the stripe order still grows after
+    3e4 cycles, so it runs the reference's own 1e5 + 1e5 (a long test)."""
+    say(f"case PT L={L} phi=2 vs the reference's production run (64 configs, {neq}+{nmess} cycles)")
+    R, C = 20, 64
+    g = J.make_params(L, L, nreplica=R, nconf=C, iconf0=1001, neq=neq, nmess=nmess, seed=2718)
+    eng = J.Engine([g])
+    start = np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R)
+    for c in range(C):
+        eng.upload_disorder(c, J.make_disorder(L, L, 3.0, 1001 + c, True, 2.0))
+        eng.set_spins(c, start)
+    assert eng.sweep_path()[0] == (2 if L % 8 else 1)
+    eng.run(J.PHASE_EQUIL, g.neq)
+    eng.run(J.PHASE_MEAS, g.nmess)
+    bf = np.stack([eng.fetch(c)["basefields"] for c in range(C)])          # config, field, slot
+    eng.close()
+    gd = os.path.join(golden_dir, "ref_prod_pt")
+    zs = []
+    for name, f, pref, col, ecol in PROD_QUANTITIES:
+        tab = read_table(find(gd, f"{pref}{L:04d}{L:04d}"))
+        assert np.allclose(tab[:, 0], J.ladder(g), atol=5e-6)
+        ref, err_ref = tab[:, col], tab[:, ecol]
+        mu, err = bf[:, f, :].mean(0), bf[:, f, :].std(0) / np.sqrt(C)
+        z = (ref - mu) / np.sqrt(err_ref ** 2 + err ** 2)
+        zs.extend(z.tolist())
+        say(f"  {name:8s} max |z| {np.max(np.abs(z)):.2f}  T=8: ref {ref[0]:.5g} engine {mu[0]:.5g}")
+    frac = zsummary(zs, f"pt_l{L}_vs_reference_production")
+    assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
