@@ -105,6 +105,7 @@ struct Engine {
     int32_t res_nt = 256;                      // threads per block of the resident sweep (default plan)
     int32_t bit_slabs = 1;                     // z-slabs of the bit-layout lattices (default plan)
     int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
+    int32_t uniform_H = 0;                     // common H of all groups, 0 if mixed
     Plan plan;                                 // resident launches of a cycle
     std::vector<OrderSeg> group_seg;           // work-list segment of each resident group
     OrderSeg all_seg;                          // of all lattices (when all are resident)
@@ -388,7 +389,8 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
                     const ResLaunch &L = pl.launches[i];
                     if ((r = hipStreamWaitEvent(e.side[i], e.fork_ev, 0)) != hipSuccess) break;
                     r = L.kind == 2 ? launch_sweep_bits(e.st, L.ord0, L.nord, L.L, L.H, L.slabs, e.shadow, e.side[i])
-                                    : launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.nt, e.shadow, e.side[i]);
+                                    : launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.H, L.nt, e.shadow,
+                                                            e.side[i]);
                     if (r != hipSuccess) break;
                     if ((r = hipEventRecord(e.join_ev[i], e.side[i])) != hipSuccess) break;
                     r = hipStreamWaitEvent(e.stream, e.join_ev[i], 0);
@@ -401,8 +403,8 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
                 auto fn = [](Engine &e, const void *a) {
                     const ResLaunch &r = *static_cast<const ResLaunch *>(a);
                     return r.kind == 2 ? launch_sweep_bits(e.st, r.ord0, r.nord, r.L, r.H, r.slabs, e.shadow, e.stream)
-                                       : launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.nt, e.shadow,
-                                                               e.stream);
+                                       : launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.H, r.nt,
+                                                               e.shadow, e.stream);
                 };
                 const size_t i = (size_t)(&L - P.launches.data());
                 if (probe && i < probe->per.size()) HIPCHK(hipEventRecord(probe->per[i].first, E.stream));
@@ -507,7 +509,9 @@ ResLaunch group_launch(const Engine &E, int g, int32_t nt, int32_t slabs = 1)
 
 ResLaunch all_launch(const Engine &E, int32_t nt)
 {
-    return {E.all_seg.ord0, E.all_seg.nord, E.uniform_L, E.max_n, nt, E.st.nlat};
+    ResLaunch r{E.all_seg.ord0, E.all_seg.nord, E.uniform_L, E.max_n, nt, E.st.nlat};
+    r.H = E.uniform_L ? E.uniform_H : 0;
+    return r;
 }
 
 Plan default_plan(const Engine &E)
@@ -917,8 +921,11 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     }
     E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_byte ? 1 : (E->all_resident ? 3 : 2));
     E->uniform_L = c0.L;
-    for (const Group &G : E->groups)
+    E->uniform_H = c0.H;
+    for (const Group &G : E->groups) {
         if (G.p.L != c0.L) E->uniform_L = 0;
+        if (G.p.H != c0.H) E->uniform_H = 0;
+    }
     if (const char *nt = std::getenv("J1J2_RES_THREADS")) {
         // a fixed block size for every resident launch, no autotuning
         const int v = std::atoi(nt);

@@ -403,8 +403,9 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     *P.own = S ^ flip;
 }
 
-// One colour pass over plane c.
-template <int WPH, int NT, int HX, bool SHADOW>
+// One colour pass over plane c.  HC > 0: the lattice height is HC, known at
+// compile time.
+template <int WPH, int NT, int HX, bool SHADOW, int HC = 0>
 __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
                                             unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
@@ -446,21 +447,37 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
             opaque(P1.own); opaque(P1.yp); opaque(P1.ym); opaque(P1.zp); opaque(P1.zm); opaque(P1.xa);
             opaque(P1.xb); opaque(P1.d1a); opaque(P1.d1b); opaque(P1.d2a); opaque(P1.d2b);
         }
-        for (int k = 0; k < nk; k += TRIP) {
+        if constexpr (HC > 0 && HC % ZSTEP == 0 && HC / ZSTEP <= 12) {
+            // every thread takes NK = HC / ZSTEP steps (its first layer z0 <
+            // ZSTEP): all of them unrolled, so every step's LDS addresses are
+            // the first step's plus an immediate, and only the first and the
+            // last step can meet the z boundary
+            constexpr int NK = HC / ZSTEP;
 #pragma unroll
-            for (int i = 0; i < TRIP; ++i) {
-                const int kk = k + i;
-                if (kk < nk) {
-                    const int z = z0 + kk * ZSTEP;
-                    NbPtr Q = (ODD && (i & 1)) ? nb_shift(P1, (i - 1) * STEP) : nb_shift(P, i * STEP);
-                    if (z == H - 1) Q.zp -= H * PER_Z;     // periodic in z
-                    if (z == 0) Q.zm += H * PER_Z;
-                    update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q, sh_prop,
-                                                 sh_mis);
-                }
+            for (int kk = 0; kk < NK; ++kk) {
+                const int z = z0 + kk * ZSTEP;
+                NbPtr Q = (ODD && (kk & 1)) ? nb_shift(P1, (kk - 1) * STEP) : nb_shift(P, kk * STEP);
+                if (kk == NK - 1 && z == HC - 1) Q.zp -= HC * PER_Z;
+                if (kk == 0 && z == 0) Q.zm += HC * PER_Z;
+                update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q, sh_prop, sh_mis);
             }
-            P = nb_shift(P, TRIP * STEP);
-            if constexpr (ODD) P1 = nb_shift(P1, TRIP * STEP);
+        } else {
+            for (int k = 0; k < nk; k += TRIP) {
+#pragma unroll
+                for (int i = 0; i < TRIP; ++i) {
+                    const int kk = k + i;
+                    if (kk < nk) {
+                        const int z = z0 + kk * ZSTEP;
+                        NbPtr Q = (ODD && (i & 1)) ? nb_shift(P1, (i - 1) * STEP) : nb_shift(P, i * STEP);
+                        if (z == H - 1) Q.zp -= H * PER_Z;     // periodic in z
+                        if (z == 0) Q.zm += H * PER_Z;
+                        update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q,
+                                                     sh_prop, sh_mis);
+                    }
+                }
+                P = nb_shift(P, TRIP * STEP);
+                if constexpr (ODD) P1 = nb_shift(P1, TRIP * STEP);
+            }
         }
     } else {
         for (int t = tid; t < nq; t += NT) {
@@ -473,11 +490,11 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
     }
 }
 
-template <int WPH, int NT, bool SHADOW>
+template <int WPH, int NT, bool SHADOW, int HC = 0>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
                                               uint32_t *lds32, int64_t (*red)[OBS_N])
 {
-    const int H = d.H;
+    const int H = HC > 0 ? HC : d.H;
     const int tid = threadIdx.x;
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
@@ -538,8 +555,8 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
-            if (c & 1) colour_pass<WPH, NT, 1, SHADOW>(lds32, cx, cc, c, H, sh_prop, sh_mis);
-            else colour_pass<WPH, NT, 0, SHADOW>(lds32, cx, cc, c, H, sh_prop, sh_mis);
+            if (c & 1) colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, c, H, sh_prop, sh_mis);
+            else colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, c, H, sh_prop, sh_mis);
             __syncthreads();
         }
     }
@@ -604,8 +621,9 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 
 // One geometry.  At most 128 VGPRs (four waves per SIMD): e.g. four 256-thread
 // blocks, i.e. four L = 32 lattices (32 KiB of LDS each), or one 1024-thread
-// block stay resident on every CU.
-template <int WPH, int NT, bool SHADOW>
+// block stay resident on every CU.  HC > 0: every lattice of the launch has
+// height HC (cubic lattices, H = L).
+template <int WPH, int NT, bool SHADOW, int HC = 0>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
 void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
 {
@@ -615,7 +633,7 @@ void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
     const int lat = st.order[ord0 + blockIdx.x];
     if (lat < 0) return;
     const LatDesc d = st.lats[lat];
-    resident_body<WPH, NT, SHADOW>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4));
+    resident_body<WPH, NT, SHADOW, HC>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4));
 }
 
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
@@ -1007,9 +1025,22 @@ size_t resident_lds_limit()
 
 using ResKernel = void (*)(DeviceState, int32_t, int32_t);
 
+// cubic: every lattice of the launch has H = L (the FP32 kernels then take
+// the height as a compile-time constant)
 template <int NT, bool SHADOW>
-ResKernel resident_kernel(int32_t L)
+ResKernel resident_kernel(int32_t L, bool cubic)
 {
+    if (cubic && !SHADOW) {
+        switch (L) {
+            case 8: return k_sweep_resident<1, NT, false, 8>;
+            case 16: return k_sweep_resident<2, NT, false, 16>;
+            case 24: return k_sweep_resident<3, NT, false, 24>;
+            case 32: return k_sweep_resident<4, NT, false, 32>;
+            case 40: return k_sweep_resident<5, NT, false, 40>;
+            case 48: return k_sweep_resident<6, NT, false, 48>;
+            default: break;
+        }
+    }
     switch (L) {
         case 8: return k_sweep_resident<1, NT, SHADOW>;
         case 16: return k_sweep_resident<2, NT, SHADOW>;
@@ -1021,13 +1052,13 @@ ResKernel resident_kernel(int32_t L)
     }
 }
 
-ResKernel pick_resident(int32_t L, int32_t nt, bool shadow)
+ResKernel pick_resident(int32_t L, int32_t nt, bool shadow, bool cubic = false)
 {
     switch (nt) {
-        case 256: return shadow ? resident_kernel<256, true>(L) : resident_kernel<256, false>(L);
-        case 512: return shadow ? resident_kernel<512, true>(L) : resident_kernel<512, false>(L);
-        case 576: return shadow ? resident_kernel<576, true>(L) : resident_kernel<576, false>(L);
-        case 1024: return shadow ? resident_kernel<1024, true>(L) : resident_kernel<1024, false>(L);
+        case 256: return shadow ? resident_kernel<256, true>(L, cubic) : resident_kernel<256, false>(L, cubic);
+        case 512: return shadow ? resident_kernel<512, true>(L, cubic) : resident_kernel<512, false>(L, cubic);
+        case 576: return shadow ? resident_kernel<576, true>(L, cubic) : resident_kernel<576, false>(L, cubic);
+        case 1024: return shadow ? resident_kernel<1024, true>(L, cubic) : resident_kernel<1024, false>(L, cubic);
         default: return nullptr;
     }
 }
@@ -1036,13 +1067,14 @@ hipError_t setup_resident(size_t max_lds)
 {
     for (int L : {8, 16, 24, 32, 40, 48, 0})
         for (int nt : {256, 512, 576, 1024})
-            for (bool sh : {false, true}) {
-                const ResKernel k = pick_resident(L, nt, sh);
-                if (!k) continue;
-                hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                   (int)(max_lds + RES_STATIC_LDS));
-                if (e != hipSuccess) return e;
-            }
+            for (bool sh : {false, true})
+                for (bool cubic : {false, true}) {
+                    const ResKernel k = pick_resident(L, nt, sh, cubic);
+                    if (!k) continue;
+                    hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)(max_lds + RES_STATIC_LDS));
+                    if (e != hipSuccess) return e;
+                }
     return hipSuccess;
 }
 
@@ -1051,10 +1083,10 @@ bool resident_nt_valid(int32_t L, int32_t nt) { return pick_resident(L, nt, fals
 bool resident_supported(int32_t L) { return L >= 8 && L <= 48 && (L % 8) == 0; }
 
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord, int32_t max_n,
-                                 int32_t L, int32_t nt, bool shadow, hipStream_t s)
+                                 int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s)
 {
     if (nord <= 0) return hipSuccess;
-    const ResKernel k = pick_resident(L, nt, shadow);
+    const ResKernel k = pick_resident(L, nt, shadow, L > 0 && H == L);
     if (!k) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, dim3(nord), dim3(nt), (size_t)max_n + RES_STATIC_LDS, s, st, ord0, nord);
     return hipGetLastError();
