@@ -191,7 +191,7 @@ int  j1j2_sweep_path(void *handle, int32_t *path, double *bytes_per_update);
  * threads per block and the lattice count, whether the launches run on
  * parallel graph branches, and whether the autotuner picked it.  The engine
  * tunes on the first call of j1j2_run / j1j2_run_cycles long enough for the
- * candidates (at most 18 cycles, counted in the call's cycles); the result of
+ * candidates (at most 36 cycles, counted in the call's cycles); the result of
  * a run does not depend on the plan.  No counterpart in the reference. */
 int  j1j2_launch_plan(void *handle, char *buf, int32_t len);
 int  j1j2_synchronize(void *handle);

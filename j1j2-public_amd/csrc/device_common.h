@@ -36,11 +36,9 @@ struct ConfDesc {         // one disorder config
     int64_t flip_off;     // flip-count offset (nrep*N)
 };
 
-struct DeviceClock {      // advanced by the last block of the PT step of each cycle
-    uint64_t cycle;       // cycles completed since the engine was created
-    uint64_t phase_start; // cycle at which the current phase began
-    uint32_t done;        // PT-step blocks finished in the current cycle
-    uint32_t pad;
+struct DeviceClock {      // counted by the PT step of every cycle
+    uint64_t ticks;       // PT-step blocks finished since the engine was created (nconf per cycle)
+    uint64_t phase_start; // ticks when the current phase began (a multiple of nconf)
 };
 
 // Per-lattice integer observables written after every cycle.

@@ -599,8 +599,10 @@ int tune_cycle(Engine &E, int measure, const Plan &P, std::vector<float> &launch
             HIPCHK(hipEventCreate(&ev.second));
             probe.per.push_back(ev);
         }
-    rc = 0;
-    if (hipEventRecord(start, E.stream) != hipSuccess) rc = fail(J1J2_EDEVICE, "hipEventRecord failed");
+    // a lead-in cycle of the same plan right before the timed one, so that
+    // the timed launches start on a busy GPU as they do in a run
+    rc = enqueue_cycle(E, measure, &P);
+    if (!rc && hipEventRecord(start, E.stream) != hipSuccess) rc = fail(J1J2_EDEVICE, "hipEventRecord failed");
     if (!rc) rc = enqueue_cycle(E, measure, &P, &probe);
     if (!rc && hipStreamSynchronize(E.stream) != hipSuccess) rc = fail(J1J2_EDEVICE, "hipStreamSynchronize failed");
     launch_ms.assign(probe.per.size(), 0.0f);
@@ -630,9 +632,11 @@ std::string tune_key(const Engine &E)
     return k;
 }
 
-// Runs up to TUNE_CYCLES cycles of the call (counted in `done`) when they fit.
+// Times the candidates on cycles of the call (counted in `done`) when all of
+// them fit: at most TUNE_CYCLES x TUNE_REPS x (4 + 5) = 36.
 constexpr int TUNE_REPS = 2;
-constexpr float TUNE_TIE = 1.02f;
+constexpr int TUNE_CYCLES = 2;        // cycles per timing: a lead-in and the timed one
+constexpr float TUNE_TIE = 1.01f;
 
 int tune(Engine &E, int measure, int ncycles, int &done)
 {
@@ -671,9 +675,10 @@ int tune(Engine &E, int measure, int ncycles, int &done)
     static constexpr int32_t ALL_NT[] = {256, 512, 576, 1024};     // one launch over all lattices
     constexpr int n_all = (int)(sizeof(ALL_NT) / sizeof(ALL_NT[0]));
     // cycles the candidates below may take (an upper bound: the call must hold them all)
-    const int need = TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_byte ? n_all : 0)) : 0);
+    const int need = TUNE_CYCLES * (TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_byte ? n_all : 0)) : 0));
     if (ncycles - done < need) return 0;             // too short a call; tune on a later one
     int rc;
+    const bool log = std::getenv("J1J2_TUNE_LOG") != nullptr;   // each timed candidate to stderr
     // 1. threads per block of each group, its launch timed alone
     std::vector<std::vector<float>> best(R.size());
     for (size_t i = 0; i < R.size(); ++i) best[i].assign(cand[i].size(), 1e30f);
@@ -688,8 +693,16 @@ int tune(Engine &E, int measure, int ncycles, int &done)
             std::vector<float> ms;
             float phase = 0.0f;
             if ((rc = tune_cycle(E, measure, P, ms, phase))) return rc;
-            ++done;
+            done += TUNE_CYCLES;
             for (size_t i = 0; i < R.size(); ++i) best[i][pick[i]] = std::min(best[i][pick[i]], ms[i]);
+            if (log)
+                for (size_t i = 0; i < R.size(); ++i)
+                    std::fprintf(stderr, "j1j2 tune: rep %d launch %s: %.4f ms\n", rep,
+                                 P.launches[i].nt ? ("L" + std::to_string(P.launches[i].L) + " nt" +
+                                                     std::to_string(P.launches[i].nt)).c_str()
+                                                  : ("L" + std::to_string(P.launches[i].L) + " s" +
+                                                     std::to_string(P.launches[i].slabs)).c_str(),
+                                 ms[i]);
         }
     Plan per;
     float t_seq = 0.0f;
@@ -724,8 +737,9 @@ int tune(Engine &E, int measure, int ncycles, int &done)
                 std::vector<float> ms;
                 float phase = 0.0f;
                 if ((rc = tune_cycle(E, measure, P, ms, phase))) return rc;
-                ++done;
+                done += TUNE_CYCLES;
                 tmin = std::min(tmin, phase);
+                if (log) std::fprintf(stderr, "j1j2 tune: rep %d %s: %.4f ms\n", rep, P.describe().c_str(), phase);
             }
             if (tmin * TUNE_TIE < t_best) {
                 t_best = tmin;
@@ -977,6 +991,10 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         return rc;
     st.lats = lats; st.confs = confs; st.cconst = cconst;
     st.nlat = (int32_t)nlat; st.nconf = (int32_t)nconf; st.nsites = E->nsites;
+    st.uni_nrep = c0.nreplica;
+    st.uni_n = E->groups[0].N;
+    for (const Group &G : E->groups)
+        if (G.p.nreplica != c0.nreplica || G.N != E->groups[0].N) st.uni_nrep = st.uni_n = 0;
     st.kd = kd; st.p64 = p64; st.qb = q; st.beta = beta; st.b2 = b2;
     st.series_cap = series_cap;
     st.ptswapever = c0.ptswapever; st.ptmeasindex = c0.ptmeasindex; st.bins = c0.bins;

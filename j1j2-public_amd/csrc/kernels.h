@@ -26,6 +26,7 @@ struct DeviceState {
     const ConfDesc *confs;     // nconf
     const ConfConst *cconst;   // nconf
     int32_t nlat, nconf;
+    int32_t uni_nrep, uni_n;   // every config has uni_nrep replicas of uni_n sites (0: they differ)
     int64_t nsites;            // sum over configs of N (per-site planes)
     const int32_t *order;      // resident launches: lattice of each block (-1: none), per launch segment
     // lattice state
@@ -118,7 +119,8 @@ hipError_t launch_pt_step(const DeviceState &st, int32_t mode, hipStream_t s);
 // locspin/locnem accumulation for the replica at slot PTMEASINDEX (of every
 // config, or with only_pending of the configs whose accumulation is pending).
 hipError_t launch_locnem(const DeviceState &st, int32_t max_n, int32_t only_pending, hipStream_t s);
-// clk->phase_start = clk->cycle
+// A phase begins at the current cycle: clk->phase_start = clk->ticks (the
+// swap alternation restarts with the odd pairs).
 hipError_t launch_begin_phase(const DeviceState &st, hipStream_t s);
 
 }  // namespace j1j2

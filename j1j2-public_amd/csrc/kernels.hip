@@ -537,7 +537,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
-    const uint64_t S0 = st.clk->cycle * (uint64_t)st.ptswapever;
+    const uint64_t S0 = clock_cycle(st) * (uint64_t)st.ptswapever;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
     // the previous cycle measured and left this replica at slot PTMEASINDEX:
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(GLB_THREADS) void k_sweep_global(DeviceState st, in
     const int slot = st.temp_of_rep[d.slot_base + d.rep];
     const float b2 = st.b2[d.slot_base + slot];
     const double beta = st.beta[d.slot_base + slot];
-    const uint64_t sweep = st.clk->cycle * (uint64_t)st.ptswapever + (uint64_t)k;
+    const uint64_t sweep = clock_cycle(st) * (uint64_t)st.ptswapever + (uint64_t)k;
     const U4 rn = philox4x32_10((uint32_t)g, (uint32_t)c | ((uint32_t)d.rep << 2), (uint32_t)sweep,
                                 (uint32_t)(sweep >> 32), st.seed, (uint32_t)cd.iconf);
     const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
@@ -820,12 +820,31 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     __shared__ int32_t rat[MAX_NREP], tor[MAX_NREP];
     __shared__ double et[MAX_NREP], vv[MAX_NREP][5];
     const int conf = blockIdx.x, tid = threadIdx.x;
-    const ConfDesc cd = st.confs[conf];
+    // the config's offsets: computed when every config has the same shape
+    // (nothing on the critical path waits for its descriptor), else loaded
+    int nrep, sb, lat0, pair_base;
+    int64_t hist_off;
+    double N;
+    if (st.uni_nrep > 0) {
+        nrep = st.uni_nrep;
+        sb = conf * nrep;
+        lat0 = sb;
+        pair_base = conf * (nrep - 1);
+        hist_off = (int64_t)sb * st.bins;
+        N = (double)st.uni_n;
+    } else {
+        const ConfDesc cd = st.confs[conf];
+        nrep = cd.nrep;
+        sb = cd.slot_base;
+        lat0 = cd.lat0;
+        pair_base = cd.pair_base;
+        hist_off = cd.hist_off;
+        N = (double)cd.N;
+    }
     const ConfConst cc = st.cconst[conf];
-    const int nrep = cd.nrep, sb = cd.slot_base;
-    const double N = (double)cd.N;
-    const uint64_t cycle = st.clk->cycle;
-    const int phase = ((cycle - st.clk->phase_start) & 1ull) ? 2 : 1;
+    const uint64_t ticks = st.clk->ticks;
+    const uint64_t cycle = ticks / (uint64_t)st.nconf;
+    const int phase = (((ticks - st.clk->phase_start) / (uint64_t)st.nconf) & 1ull) ? 2 : 1;
     const int64_t slen = st.local ? st.series_len[conf] : 0;
     // The kernel is a chain of dependent memory round trips; everything that
     // does not depend on this cycle's energies is requested up front: the
@@ -847,7 +866,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
         hbin0 = st.hbin[sb + tid];
     }
     for (int r = tid; r < nrep; r += PT_THREADS) {
-        const int64_t *o = st.obs + (int64_t)(cd.lat0 + r) * OBS_N;
+        const int64_t *o = st.obs + (int64_t)(lat0 + r) * OBS_N;
         const double ej = ((cc.J1h * (double)o[OBS_BX]) + (cc.J1v * (double)o[OBS_BY])) +
                           ((cc.J2 * (double)o[OBS_BD]) + (cc.J3 * (double)o[OBS_BZ]));
         const double E = -ej - (double)o[OBS_EQ] * 0x1p-32;
@@ -866,14 +885,14 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     if (!st.local && swaps) {
         for (int s = s0; s < nrep - 1; s += 2 * PT_THREADS) {
             const int rlo = rat[s], rhi = rat[s + 1];
-            atomicAdd(reinterpret_cast<unsigned long long *>(st.att + cd.pair_base + s), 1ull);
+            atomicAdd(reinterpret_cast<unsigned long long *>(st.att + pair_base + s), 1ull);
             const double blo = s == s0 ? b_lo : st.beta[sb + s], bhi = s == s0 ? b_hi : st.beta[sb + s + 1];
             const double delta = (blo - bhi) * (et[rlo] - et[rhi]);
             const U4 rn = philox4x32_10((uint32_t)s, 0x80000000u, (uint32_t)cycle, (uint32_t)(cycle >> 32),
-                                        st.seed, (uint32_t)cd.iconf);
+                                        st.seed, (uint32_t)st.confs[conf].iconf);
             const double u = (double)rn.x * 0x1p-32;
             if (delta >= 0.0 || u < exp64(delta)) {
-                atomicAdd(reinterpret_cast<unsigned long long *>(st.acc + cd.pair_base + s), 1ull);
+                atomicAdd(reinterpret_cast<unsigned long long *>(st.acc + pair_base + s), 1ull);
                 rat[s] = rhi;
                 rat[s + 1] = rlo;
                 tor[rlo] = s + 1;
@@ -896,7 +915,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
             const double am = vv[r][0], ae = vv[r][1], ax = vv[r][2], ay = vv[r][3], an = vv[r][4];
             if (st.local) {
                 if (slen < st.series_cap) {
-                    double *sr = st.series + cd.series_off + (int64_t)sl * 5 * st.series_cap;
+                    double *sr = st.series + st.confs[conf].series_off + (int64_t)sl * 5 * st.series_cap;
                     for (int q = 0; q < 5; ++q) sr[(int64_t)q * st.series_cap + slen] = vv[r][q];
                 }
                 continue;
@@ -937,7 +956,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
             else if (tq >= (double)st.bins) idx = st.bins;
             else idx = 1 + (int)tq;
             idx = idx < 1 ? 1 : (idx > st.bins ? st.bins : idx);
-            atomicAdd(reinterpret_cast<unsigned long long *>(st.hist + cd.hist_off + (int64_t)sl * st.bins + idx - 1),
+            atomicAdd(reinterpret_cast<unsigned long long *>(st.hist + hist_off + (int64_t)sl * st.bins + idx - 1),
                       1ull);
         }
         if (tid == 0) {
@@ -945,20 +964,13 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
             if (st.local) st.series_len[conf] = slen + 1;
         }
     }
-    if (swaps) {
-        if (tid == 0 && st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
-        // the last block to finish advances the cycle clock: every thread of
-        // every block has used its read of the clock before its block's
-        // barrier and increment of `done`, and the next kernel sees the new
-        // value across the launch boundary, so no fence is needed
-        __syncthreads();
-        if (tid == 0) {
-            const unsigned prev = atomicAdd(&st.clk->done, 1u);
-            if (prev == gridDim.x - 1) {
-                st.clk->done = 0;
-                st.clk->cycle = cycle + 1;
-            }
-        }
+    if (swaps && tid == 0) {
+        if (st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
+        // one tick per block (nconf per cycle, see clock_cycle): the block's
+        // single wave read the clock above, and the next kernel sees every
+        // tick across the launch boundary, so no fence and no returned value
+        // are needed
+        atomicAdd(reinterpret_cast<unsigned long long *>(&st.clk->ticks), 1ull);
     }
 }
 
@@ -1007,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
     if (blockIdx.x == 0 && threadIdx.x == 0) st.locsamples[conf] += 1;
 }
 
-__global__ void k_begin_phase(DeviceClock *clk) { clk->phase_start = clk->cycle; }
+__global__ void k_begin_phase(DeviceClock *clk) { clk->phase_start = clk->ticks; }
 
 // reduction scratch after the lattice in the resident sweep's dynamic LDS
 constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (1024 / 64) * OBS_N;

@@ -8,6 +8,14 @@
 
 namespace j1j2 {
 namespace {
+// The current PT cycle.  Every block of the PT step adds one tick when it is
+// done, nconf per cycle; kernels between PT steps see a multiple of nconf,
+// and a PT-step block reads the clock before its own tick, so it sees fewer
+// than nconf ticks of its own cycle.
+__device__ __forceinline__ uint64_t clock_cycle(const DeviceState &st)
+{
+    return st.clk->ticks / (uint64_t)st.nconf;
+}
 
 __device__ __forceinline__ uint32_t lds_bit(uint32_t w, int j) { return (w >> (8 * j + 2)) & 1u; }
 
