@@ -26,6 +26,7 @@ sys.path.insert(0, os.path.join(REPO, "j1j2-public_amd"))
 
 L, NREP, NCONF, PTSWAPEVER = 32, 16, 64, 5
 HBM_PEAK = 8.0e12                  # MI355X HBM3E, bytes/s
+TUNE_CYCLES = 40                   # the autotuner times its candidates on at most 36 cycles of one call
 SEED = 20261203
 
 
@@ -49,6 +50,7 @@ def time_form(J, precision, steps, warmup, kernel_timing, nconf=NCONF):
     launch with HIP events on the engine's stream (launched eagerly) for the
     sweep kernel's average duration."""
     eng = build_batch(J, precision, nconf=nconf)
+    eng.run(J.PHASE_EQUIL, TUNE_CYCLES)      # the autotuner's cycles, before any warmup or timed step
     eng.run(J.PHASE_EQUIL, warmup)
     eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
     eng.synchronize()
@@ -168,6 +170,7 @@ def build_groups(J, groups, precision, flags=0, seed=SEED):
 
 def time_groups(J, groups, precision, steps, warmup, flags=0):
     eng = build_groups(J, groups, precision, flags)
+    eng.run(J.PHASE_EQUIL, TUNE_CYCLES)      # the autotuner's cycles, before any warmup or timed step
     eng.run(J.PHASE_EQUIL, warmup)
     eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
     eng.synchronize()
