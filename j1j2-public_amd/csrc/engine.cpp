@@ -106,6 +106,7 @@ struct Engine {
     int32_t bit_slabs = 1;                     // z-slabs of the bit-layout lattices (default plan)
     int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
     int32_t uniform_H = 0;                     // common H of all groups, 0 if mixed
+    bool all_cubic = false;                    // every group has H = L
     Plan plan;                                 // resident launches of a cycle
     std::vector<OrderSeg> group_seg;           // work-list segment of each resident group
     OrderSeg all_seg;                          // of all lattices (when all are resident)
@@ -510,7 +511,7 @@ ResLaunch group_launch(const Engine &E, int g, int32_t nt, int32_t slabs = 1)
 ResLaunch all_launch(const Engine &E, int32_t nt)
 {
     ResLaunch r{E.all_seg.ord0, E.all_seg.nord, E.uniform_L, E.max_n, nt, E.st.nlat};
-    r.H = E.uniform_L ? E.uniform_H : 0;
+    r.H = E.uniform_L ? E.uniform_H : (E.all_cubic ? -1 : 0);
     return r;
 }
 
@@ -936,9 +937,11 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_byte ? 1 : (E->all_resident ? 3 : 2));
     E->uniform_L = c0.L;
     E->uniform_H = c0.H;
+    E->all_cubic = true;
     for (const Group &G : E->groups) {
         if (G.p.L != c0.L) E->uniform_L = 0;
         if (G.p.H != c0.H) E->uniform_H = 0;
+        if (G.p.H != G.p.L) E->all_cubic = false;
     }
     if (const char *nt = std::getenv("J1J2_RES_THREADS")) {
         // a fixed block size for every resident launch, no autotuning

@@ -77,7 +77,8 @@ struct DeviceState {
 // a multiple of 8; -1 entries are idle).  Requires resident_supported(L) for
 // every lattice and resident_nt_valid(L, nt).  L: the common lattice size of
 // the segment, or 0 for mixed sizes; H: the common height when L > 0 (0 if
-// the heights differ).  The result depends neither on nt nor on the order.
+// the heights differ), and for mixed sizes -1 if every lattice has H = L (0
+// otherwise).  The result depends neither on nt nor on the order.
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord,
                                  int32_t max_n, int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);

@@ -238,6 +238,15 @@ constexpr int fast_zs()
     return (ZS >= 2 && 4 * ZS * PER_Z >= 3 * NT) ? ZS : 0;
 }
 
+// With the lattice height HC known at compile time, a colour pass of the
+// fixed-geometry stepping unrolls all of its ceil(HC / ZS) steps (at most 12).
+template <int WPH, int NT, int HC>
+constexpr bool unrolled_pass()
+{
+    constexpr int ZS = fast_zs<WPH, NT>();
+    return HC > 0 && ZS >= 2 && (HC + ZS - 1) / ZS <= 12;
+}
+
 template <int WPH, int NT, int HX>
 __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const QPlanes &qp,
                                              ObsFast &a)
@@ -447,15 +456,18 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
             opaque(P1.own); opaque(P1.yp); opaque(P1.ym); opaque(P1.zp); opaque(P1.zm); opaque(P1.xa);
             opaque(P1.xb); opaque(P1.d1a); opaque(P1.d1b); opaque(P1.d2a); opaque(P1.d2b);
         }
-        if constexpr (HC > 0 && HC % ZSTEP == 0 && HC / ZSTEP <= 12) {
-            // every thread takes NK = HC / ZSTEP steps (its first layer z0 <
-            // ZSTEP): all of them unrolled, so every step's LDS addresses are
-            // the first step's plus an immediate, and only the first and the
-            // last step can meet the z boundary
-            constexpr int NK = HC / ZSTEP;
+        if constexpr (unrolled_pass<WPH, NT, HC>()) {
+            // a thread (first layer z0 < ZSTEP) takes NK = ceil(HC / ZSTEP)
+            // steps, the last one only if z0 + (NK-1) ZSTEP < HC: all of them
+            // unrolled, so every step's LDS addresses are the first step's
+            // plus an immediate.  Layer HC-1 is some thread's last step and
+            // layer 0 a first step, so only those meet the z boundary.
+            constexpr int NK = (HC + ZSTEP - 1) / ZSTEP;
+            constexpr bool RAGGED = HC % ZSTEP != 0;
 #pragma unroll
             for (int kk = 0; kk < NK; ++kk) {
                 const int z = z0 + kk * ZSTEP;
+                if (RAGGED && kk == NK - 1 && z >= HC) break;
                 NbPtr Q = (ODD && (kk & 1)) ? nb_shift(P1, (kk - 1) * STEP) : nb_shift(P, kk * STEP);
                 if (kk == NK - 1 && z == HC - 1) Q.zp -= HC * PER_Z;
                 if (kk == 0 && z == 0) Q.zm += HC * PER_Z;
@@ -637,7 +649,8 @@ void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
 }
 
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
-template <int NT, bool SHADOW>
+// CUBIC: every lattice of the launch has H = L.
+template <int NT, bool SHADOW, bool CUBIC = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
 void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
 {
@@ -647,12 +660,12 @@ void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
     const LatDesc d = st.lats[lat];
     int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4);
     switch (d.L) {
-        case 8: resident_body<1, NT, SHADOW>(st, d, lat, lds32, red); break;
-        case 16: resident_body<2, NT, SHADOW>(st, d, lat, lds32, red); break;
-        case 24: resident_body<3, NT, SHADOW>(st, d, lat, lds32, red); break;
-        case 32: resident_body<4, NT, SHADOW>(st, d, lat, lds32, red); break;
-        case 40: resident_body<5, NT, SHADOW>(st, d, lat, lds32, red); break;
-        case 48: resident_body<6, NT, SHADOW>(st, d, lat, lds32, red); break;
+        case 8: resident_body<1, NT, SHADOW, CUBIC ? 8 : 0>(st, d, lat, lds32, red); break;
+        case 16: resident_body<2, NT, SHADOW, CUBIC ? 16 : 0>(st, d, lat, lds32, red); break;
+        case 24: resident_body<3, NT, SHADOW, CUBIC ? 24 : 0>(st, d, lat, lds32, red); break;
+        case 32: resident_body<4, NT, SHADOW, CUBIC ? 32 : 0>(st, d, lat, lds32, red); break;
+        case 40: resident_body<5, NT, SHADOW, CUBIC ? 40 : 0>(st, d, lat, lds32, red); break;
+        case 48: resident_body<6, NT, SHADOW, CUBIC ? 48 : 0>(st, d, lat, lds32, red); break;
         default: break;   // the host only sends L in {8,...,48}, L % 8 == 0, here
     }
 }
@@ -1050,7 +1063,7 @@ ResKernel resident_kernel(int32_t L, bool cubic)
             case 32: return k_sweep_resident<4, NT, false, 32>;
             case 40: return k_sweep_resident<5, NT, false, 40>;
             case 48: return k_sweep_resident<6, NT, false, 48>;
-            default: break;
+            default: return k_sweep_resident_mixed<NT, false, true>;
         }
     }
     switch (L) {
@@ -1098,7 +1111,7 @@ hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t no
                                  int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s)
 {
     if (nord <= 0) return hipSuccess;
-    const ResKernel k = pick_resident(L, nt, shadow, L > 0 && H == L);
+    const ResKernel k = pick_resident(L, nt, shadow, L > 0 ? H == L : H < 0);
     if (!k) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, dim3(nord), dim3(nt), (size_t)max_n + RES_STATIC_LDS, s, st, ord0, nord);
     return hipGetLastError();

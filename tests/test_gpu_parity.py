@@ -66,12 +66,17 @@ def test_resident_block_sizes_match_oracle(nt, monkeypatch):
         eng, n = run_and_compare([g], "str" if k % 2 else "hot", 2, 3)
         assert n == 1 and f"L{g.L} nt{nt}" in eng.launch_plan()
         eng.close()
-    # cubic lattices (H = L) take kernels specialised on the height
-    for k, L in enumerate((8, 16, 24, 32, 40, 48)):
-        g = J.make_params(L, L, nreplica=2, nconf=1, iconf0=7 + k, flags=LOCNEM, seed=SEED, ptmeasindex=2)
+    # cubic lattices (H = L) take kernels specialised on the height: one
+    # launch per size, and one mixed launch of all sizes
+    cubic = [J.make_params(L, L, nreplica=2, nconf=1, iconf0=7 + k, flags=LOCNEM, seed=SEED, ptmeasindex=2)
+             for k, L in enumerate((8, 16, 24, 32, 40, 48))]
+    for k, g in enumerate(cubic):
         eng, n = run_and_compare([g], "hot" if k % 2 else "str", 2, 3)
-        assert n == 1 and f"L{L} nt{nt}" in eng.launch_plan()
+        assert n == 1 and f"L{g.L} nt{nt}" in eng.launch_plan()
         eng.close()
+    eng, n = run_and_compare(cubic, ["str", "hot"] * 3, 2, 3)
+    assert n == len(cubic) and f"L0 nt{nt}" in eng.launch_plan()
+    eng.close()
 
 
 @pytest.mark.parametrize("L,H,tmin,tmax", [(16, 8, 0.02, 0.05), (16, 8, 2.0e4, 1.0e5), (64, 4, 0.02, 0.05),
