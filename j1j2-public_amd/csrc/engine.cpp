@@ -988,6 +988,8 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
     st.slab_x = st.slab_obs = nullptr;
     st.slab_err = nullptr;
+    st.slab_spin_max = 1 << 21;                        // about seconds of polling
+    if (const char *sp = std::getenv("J1J2_SLAB_SPIN_MAX")) st.slab_spin_max = std::atoi(sp);
     if (bits_lds &&
         ((rc = E->alloc(st.slab_x, (int64_t)bits_slab_x_words((int32_t)nlat))) ||
          (rc = E->alloc(st.slab_obs, nlat * BIT_SLAB_MAX * 16)) || (rc = E->alloc(st.slab_err, 1))))

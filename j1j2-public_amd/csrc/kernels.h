@@ -67,6 +67,7 @@ struct DeviceState {
     // set when a bounded wait for a neighbour's words runs out
     unsigned long long *slab_x, *slab_obs;
     int32_t *slab_err;
+    int32_t slab_spin_max;     // polls before a wait gives up (< 0: every wait gives up at once)
     // run constants
     int32_t ptswapever, ptmeasindex, bins, local;
     uint32_t seed;

@@ -349,7 +349,6 @@ __device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, 
 constexpr int XV_SHIFT = 50;                             // a unit's HW <= 48 sites below, the version above
 constexpr unsigned long long XW_MASK = (1ull << XV_SHIFT) - 1;
 constexpr uint32_t XV_MASK = (1u << (64 - XV_SHIFT)) - 1;
-constexpr int SPIN_MAX = 1 << 21;                        // polls before a wait gives up (seconds)
 
 __device__ __forceinline__ unsigned long long xload(const unsigned long long *p)
 {
@@ -362,14 +361,21 @@ __device__ __forceinline__ void xstore(unsigned long long *p, unsigned long long
 }
 
 // Bounded wait until the word at p carries version `ver` in its bits from
-// `shift` up; on running out (or once another wait has) the error flag is set
-// and the wait ends, so every wave always finishes.
-__device__ unsigned long long xwait(const unsigned long long *p, int shift, uint32_t ver, int32_t *err)
+// `shift` up; after spin_max polls (or once another wait has run out) the
+// error flag is set and the wait ends, so every wave always finishes.
+// spin_max < 0 makes every wait give up at once (the host's test of the
+// error path).
+__device__ unsigned long long xwait(const unsigned long long *p, int shift, uint32_t ver, int32_t *err,
+                                    int spin_max)
 {
     unsigned long long w = xload(p);
+    if (spin_max < 0) {
+        atomicOr(err, 1);
+        return w;
+    }
     for (int it = 1; (uint32_t)(w >> shift) != ver; ++it) {
         if ((it & 255) == 0) {
-            if (it >= SPIN_MAX) {
+            if (it >= spin_max) {
                 atomicOr(err, 1);
                 break;
             }
@@ -414,7 +420,8 @@ __device__ __forceinline__ void slab_fetch(const DeviceState &st, uint32_t *lds3
     if (tid < 2 * HW) {
         const int w = tid >= HW, yh = tid - w * HW;            // w = 0: lower halo, 1: upper
         const int src = w ? (s + 1 == S ? 0 : s + 1) : (s == 0 ? S - 1 : s - 1);
-        const unsigned long long x = xwait(xword<HW>(st, lat, src, c, 1 - w, yh), XV_SHIFT, ver, st.slab_err);
+        const unsigned long long x = xwait(xword<HW>(st, lat, src, c, 1 - w, yh), XV_SHIFT, ver, st.slab_err,
+                                           st.slab_spin_max);
         reinterpret_cast<uint2 *>(lds32)[c * sh.NU + (w ? sh.nz + 1 : 0) * HW + yh] = unit_from_sites<L>(x & XW_MASK);
     }
 }
@@ -580,8 +587,9 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
             for (int q = 1; q < S; ++q)
 #pragma unroll
                 for (int k = 0; k < OBS_N; ++k) {
-                    const uint32_t lo = (uint32_t)xwait(ox + q * 16 + 2 * k, 32, ov, st.slab_err);
-                    const uint32_t hi = (uint32_t)xwait(ox + q * 16 + 2 * k + 1, 32, ov, st.slab_err);
+                    const uint32_t lo = (uint32_t)xwait(ox + q * 16 + 2 * k, 32, ov, st.slab_err, st.slab_spin_max);
+                    const uint32_t hi =
+                        (uint32_t)xwait(ox + q * 16 + 2 * k + 1, 32, ov, st.slab_err, st.slab_spin_max);
                     v[k] += (int64_t)(((uint64_t)hi << 32) | lo);
                 }
             xstore(ox, (unsigned long long)(lc + 1u));   // every slab has read lc and published

@@ -290,3 +290,22 @@ def test_pt_ladder_continues_across_calls():
     ora.run(2, False)
     ora.run(5, True)
     compare_state(eng, 0, ora, J.ladder(gg))
+
+
+def test_slab_wait_failure_is_reported(monkeypatch):
+    """The error path of the slab exchange, forced from the host: with
+    J1J2_SLAB_SPIN_MAX < 0 every bounded wait gives up at once and sets the
+    error flag, every wave still finishes, and the run returns J1J2_EDEVICE
+    with its message instead of hanging; a later engine runs normally."""
+    say("case slab wait failure forced from the host")
+    g = J.make_params(64, 8, nreplica=2, nconf=2, iconf0=3, seed=SEED)
+    monkeypatch.setenv("J1J2_BIT_SLABS", "2")
+    monkeypatch.setenv("J1J2_SLAB_SPIN_MAX", "-1")
+    eng, _ = build_engine([g], "hot")
+    assert "s2" in eng.launch_plan()
+    with pytest.raises(J.EngineError) as e:
+        eng.run(J.PHASE_EQUIL, 1)
+    assert e.value.code == -3 and "slab" in str(e.value)
+    eng.close()
+    monkeypatch.delenv("J1J2_SLAB_SPIN_MAX")
+    run_and_compare([g], "hot", 1, 1)
