@@ -11,6 +11,8 @@
 #       phi = 2, xi = 3, 20 replicas over T = 8.0..11.5, STR-X start,
 #       NCONF = 1000, NEQ = NMESS = 100000 cycles, PTSWAPEVER = 5
 #       (cfen, cfma, cfst, cfnm)
+#   tests/golden/ref_prod_pt_hot/ <- the same with HOT starts
+#       (J2-1_P0.00_RAN_HOT_s100_scanpt_prod_main/L{20,24}; L = 24: 900 configs)
 # Needs /root/reference (present in the build container, not on GPU hosts).
 set -euo pipefail
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -30,3 +32,4 @@ copy() {   # source dir, output dir, sizes, tables
 }
 copy "$RAW/p0/c3/J2-1_P0.00_RAN_STRX_s100_scanclean_no_field" "$HERE/ref_prod_clean" "20 40" "cfst"
 copy "$RAW/p2/c3/J2-1_P0.00_RAN_STRX_s100_scanpt_prod_main" "$HERE/ref_prod_pt" "20 24 40" "cfen cfma cfst cfnm"
+copy "$RAW/p2/c3/J2-1_P0.00_RAN_HOT_s100_scanpt_prod_main" "$HERE/ref_prod_pt_hot" "20 24" "cfen cfma cfst cfnm"

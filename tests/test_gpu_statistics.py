@@ -25,7 +25,7 @@ import pytest
 
 import j1j2_engine as J
 import oracle as O
-from gpu_cases import say
+from gpu_cases import config_inputs, say
 from refdata import find, read_table
 
 pytestmark = pytest.mark.gpu
@@ -232,9 +232,10 @@ LONG = pytest.mark.skipif(not os.environ.get("J1J2_LONG_TESTS"),
                           reason="long run (~90 s on one MI355X); set J1J2_LONG_TESTS=1")
 
 
-@pytest.mark.parametrize("L,neq,nmess", [(20, 2000, 4000), (24, 2000, 4000),
-                                         pytest.param(40, 100000, 100000, marks=LONG)])
-def test_pt_matches_reference_production(golden_dir, L, neq, nmess):
+@pytest.mark.parametrize("L,neq,nmess,start", [(20, 2000, 4000, "str"), (24, 2000, 4000, "str"),
+                                               (20, 2000, 4000, "hot"), (24, 2000, 4000, "hot"),
+                                               pytest.param(40, 100000, 100000, "str", marks=LONG)])
+def test_pt_matches_reference_production(golden_dir, L, neq, nmess, start):
     """The reference's own production PT runs (data/raw/corr_random_field/p2/c3/
     J2-1_P0.00_RAN_STRX_s100_scanpt_prod_main/L20, L24, L40, copied to
     tests/golden/ref_prod_pt with their run_meta.txt): PT SERIAL, phi = 2,
@@ -249,21 +250,24 @@ def test_pt_matches_reference_production(golden_dir, L, neq, nmess):
     reports |time average of m|, which shrinks with the run length.)  L = 20
     and 24 equilibrate within 2000 + 4000 cycles; at L = 40 the swap
     acceptance is ~0.2% at the cold end and the stripe order still grows after
-    3e4 cycles, so it runs the reference's own 1e5 + 1e5 (a long test)."""
-    say(f"case PT L={L} phi=2 vs the reference's production run (64 configs, {neq}+{nmess} cycles)")
+    3e4 cycles, so it runs the reference's own 1e5 + 1e5 (a long test).
+    The HOT-start runs (.../J2-1_P0.00_RAN_HOT_s100_scanpt_prod_main, fixtures
+    in tests/golden/ref_prod_pt_hot) are compared the same way, the engine
+    starting each replica from the reference's own HOT start for its seed."""
+    say(f"case PT L={L} phi=2 {start} vs the reference's production run (64 configs, {neq}+{nmess} cycles)")
     R, C = 20, 64
     g = J.make_params(L, L, nreplica=R, nconf=C, iconf0=1001, neq=neq, nmess=nmess, seed=2718)
     eng = J.Engine([g])
-    start = np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R)
     for c in range(C):
-        eng.upload_disorder(c, J.make_disorder(L, L, 3.0, 1001 + c, True, 2.0))
-        eng.set_spins(c, start)
+        p, s = config_inputs(L, L, R, 1001 + c, start)
+        eng.upload_disorder(c, p)
+        eng.set_spins(c, s.reshape(-1))
     assert eng.sweep_path()[0] == (2 if L % 8 else 1)
     eng.run(J.PHASE_EQUIL, g.neq)
     eng.run(J.PHASE_MEAS, g.nmess)
     bf = np.stack([eng.fetch(c)["basefields"] for c in range(C)])          # config, field, slot
     eng.close()
-    gd = os.path.join(golden_dir, "ref_prod_pt")
+    gd = os.path.join(golden_dir, "ref_prod_pt" if start == "str" else "ref_prod_pt_hot")
     zs = []
     for name, f, pref, col, ecol in PROD_QUANTITIES:
         tab = read_table(find(gd, f"{pref}{L:04d}{L:04d}"))
@@ -273,5 +277,5 @@ def test_pt_matches_reference_production(golden_dir, L, neq, nmess):
         z = (ref - mu) / np.sqrt(err_ref ** 2 + err ** 2)
         zs.extend(z.tolist())
         say(f"  {name:8s} max |z| {np.max(np.abs(z)):.2f}  T=8: ref {ref[0]:.5g} engine {mu[0]:.5g}")
-    frac = zsummary(zs, f"pt_l{L}_vs_reference_production")
+    frac = zsummary(zs, f"pt_l{L}_{start}_vs_reference_production")
     assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
