@@ -309,3 +309,23 @@ def test_slab_wait_failure_is_reported(monkeypatch):
     eng.close()
     monkeypatch.delenv("J1J2_SLAB_SPIN_MAX")
     run_and_compare([g], "hot", 1, 1)
+
+
+@pytest.mark.parametrize("L,H,flags,precision,slabs", [
+    (16, 16, 0, J.FP32_BATCHED, None),                  # byte layout, cubic (height-specialised)
+    (24, 8, 0, J.FP32_BATCHED, None),                   # byte layout, general height
+    (64, 8, 0, J.FP32_BATCHED, "1"),                    # bit layout, whole lattices
+    (64, 8, 0, J.FP32_BATCHED, "2"),                    # bit layout, two z-slabs
+    (10, 6, 0, J.FP32_BATCHED, None),                   # global-memory sweep (L % 8 != 0)
+    (8, 8, 0, J.FP64_PLAIN, None)])
+def test_anisotropic_couplings_match_oracle(L, H, flags, precision, slabs, monkeypatch):
+    """J1h != J1v (deltaJ1), J2 and J3 away from +-1: a coupling taken for
+    another (x for y, J2 for J3, a sign) on any sweep path breaks bit
+    equality with the oracle."""
+    say(f"case anisotropic couplings L={L} H={H} precision {precision} slabs {slabs}")
+    if slabs is not None:
+        monkeypatch.setenv("J1J2_BIT_SLABS", slabs)
+    g = J.make_params(L, H, nreplica=3, nconf=2, iconf0=9, precision=precision, flags=LOCNEM | flags,
+                      J1=1.1, deltaJ1=0.3, J2=-0.7, J3=1.3, seed=SEED, ptmeasindex=2)
+    eng, n = run_and_compare([g], "hot", 2, 3)
+    assert n == 2
