@@ -329,3 +329,17 @@ def test_anisotropic_couplings_match_oracle(L, H, flags, precision, slabs, monke
                       J1=1.1, deltaJ1=0.3, J2=-0.7, J3=1.3, seed=SEED, ptmeasindex=2)
     eng, n = run_and_compare([g], "hot", 2, 3)
     assert n == 2
+
+
+@pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
+def test_split_ladder_matches_oracle(precision):
+    """PTLADDERMODE = SPLIT (J1J2_6tv.f90:928-962): unevenly spaced
+    temperatures, so a swap that used the wrong pair's betas breaks bit
+    equality with the oracle."""
+    say(f"case SPLIT ladder precision {precision}")
+    g = J.make_params(8, 8, nreplica=5, nconf=2, iconf0=4, precision=precision, ladder=J.LADDER_SPLIT,
+                      tladdermid=9.0, tladderfrac=0.6, flags=LOCNEM, seed=SEED, ptmeasindex=3)
+    d = np.diff(J.ladder(g))
+    assert np.all(d > 0) and not np.allclose(d, d[0])
+    eng, n = run_and_compare([g], "str", 2, 3)
+    assert n == 2
