@@ -369,16 +369,24 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     uint32_t flip = 0;
     float ys[4];
     uint32_t open = 0;
+    // the four sites' fields term by term (each site's sum in the same order):
+    // independent neighbours in the issue stream, no dependent fma pairs
+    float h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = __builtin_fmaf(f16_site(xpe, xpo, j), kxpa[j], kca[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = __builtin_fmaf(f16_site(xme, xmo, j), kxma[j], h[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = __builtin_fmaf(f16_site(ype, ypo, j), kypa[j], h[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = __builtin_fmaf(f16_site(yme, ymo, j), kyma[j], h[j]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = __builtin_fmaf(f16_site(de, dod, j), cx.j2d, h[j]);  // (2^-24 4D) (2^24 2 J2 / 4)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = __builtin_fmaf(f16_site(ze, zod, j), cx.j3d, h[j]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        float h = kca[j];
-        h = __builtin_fmaf(f16_site(xpe, xpo, j), kxpa[j], h);
-        h = __builtin_fmaf(f16_site(xme, xmo, j), kxma[j], h);
-        h = __builtin_fmaf(f16_site(ype, ypo, j), kypa[j], h);
-        h = __builtin_fmaf(f16_site(yme, ymo, j), kyma[j], h);
-        h = __builtin_fmaf(f16_site(de, dod, j), cx.j2d, h);  // (2^-24 4D) (2^24 2 J2 / 4)
-        h = __builtin_fmaf(f16_site(ze, zod, j), cx.j3d, h);
-        ys[j] = y_f32_word(h, S, 8 * j + 2, cx.b2);
+        ys[j] = y_f32_word(h[j], S, 8 * j + 2, cx.b2);
         if constexpr (!SHADOW) {
             const Filt f = decide_f32_filter(ys[j], ra[j]);
             flip |= (f.acc ? 4u : 0u) << (8 * j);

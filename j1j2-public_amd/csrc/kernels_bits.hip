@@ -194,17 +194,24 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
     const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
     uint32_t f = 0, open = 0;
-    float ys[4];
+    float ys[4], h[4];
+    // the four sites' fields term by term (each site's sum in the same order):
+    // independent neighbours in the issue stream, no dependent fma pairs
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(xpe, xpo, k), kxpa[k], kca[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(xme, xmo, k), kxma[k], h[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(ype, ypo, k), kypa[k], h[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(de, dod, k), cx.j2d, h[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(ze, zod, k), cx.j3d, h[k]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float h = kca[k];
-        h = __builtin_fmaf(f16_site(xpe, xpo, k), kxpa[k], h);
-        h = __builtin_fmaf(f16_site(xme, xmo, k), kxma[k], h);
-        h = __builtin_fmaf(f16_site(ype, ypo, k), kypa[k], h);
-        h = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h);
-        h = __builtin_fmaf(f16_site(de, dod, k), cx.j2d, h);
-        h = __builtin_fmaf(f16_site(ze, zod, k), cx.j3d, h);
-        ys[k] = y_f32_word(h, s.S, k, cx.b2);
+        ys[k] = y_f32_word(h[k], s.S, k, cx.b2);
         if constexpr (!SHADOW) {
             const Filt fl = decide_f32_filter(ys[k], ra[k]);
             f |= (fl.acc ? 1u : 0u) << k;
