@@ -250,8 +250,9 @@ def main():
                     help="also run 5 plain device copies of this many bytes (PMC counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only; the path would shard disorder configs "
-              "across GPUs with no exchange during the run (see DESIGN.md).", file=sys.stderr)
+        print("bench.py: measures one GPU only; the path would shard disorder configs across GPUs "
+              "(one process per GPU, each its own engine over its share of the configs, no exchange "
+              "during the run, results gathered once at the end).", file=sys.stderr)
         sys.exit(2)
     import j1j2_engine as J
 
