@@ -18,7 +18,11 @@
 ! config and the locnem files of config 1 (Local_nematic_OP,
 ! J1J2_6tv.f90:2845-2884).  Files are rewritten after every batch with the
 ! number of configs completed so far, as the reference rewrites them after
-! every config.
+! every config.  progress.txt and the progress lines follow Write_progress
+! (J1J2_6tv.f90:499-529; a PT phase runs in chunks of a tenth, as the
+! reference reports), and the timing lines Write_timing (J1J2_6tv.f90:475-496)
+! per batch: iconf is the batch's first config and a trailing field names
+! the batch's configs.
 !
 ! Not supported (rejected with a message and stop 1): dilution (PP, PPh,
 ! PPv > 0), UPDATEORDER = 'RND', PTPARALLELMODE = 'GROUPED'.  WRITEBINDTIME
@@ -79,7 +83,8 @@ program j1j2_host
    real(r8b), allocatable :: jointp(:, :)              ! (0:L3-1, NCONF)
    integer(c_int64_t), allocatable :: sum_att(:), sum_acc(:)
    integer(c_int64_t) :: sum_visitboth = 0
-   integer(i4b) :: first, nbatch
+   integer(i4b) :: first, nbatch, last
+   integer(c_int64_t) :: tc0, tc1, tc2, trate
 
    call read_options()
    call validate_options()
@@ -109,12 +114,20 @@ program j1j2_host
    nbatch = BATCH
    if (nbatch <= 0) nbatch = NCONF
    do first = 1, NCONF, nbatch
+      last = min(NCONF, first + nbatch - 1)
+      call system_clock(tc0, trate)
+      call write_progress('CONFIG_START', first, 0, 0)
       if (SAMPLERMODE == 'PT') then
-         call run_pt_batch(first, min(nbatch, NCONF - first + 1))
+         call run_pt_batch(first, last - first + 1)
       else
-         call run_local_batch(first, min(nbatch, NCONF - first + 1))
+         call run_local_batch(first, last - first + 1)
       end if
-      call write_files(min(NCONF, first + nbatch - 1))
+      ! Collect_data + Write_files of the reference (J1J2_6tv.f90:373-384)
+      call system_clock(tc1)
+      call write_files(last)
+      call system_clock(tc2)
+      call print_timing('COLLECT_DATA', first, last, tc1, tc2, trate)
+      call print_timing('CONFIG_TOTAL', first, last, tc0, tc2, trate)
    end do
 
 contains
@@ -283,12 +296,65 @@ contains
       close (u)
    end subroutine
 
+   ! The reference's timing line (Write_timing, J1J2_6tv.f90:475-496): the
+   ! phase's seconds for the batch of configs a .. b run in one engine, so
+   ! iconf is the batch's first config and the configs of the batch follow.
    subroutine print_timing(label, a, b, t0, t1, rate)
       character(len=*), intent(in) :: label
       integer, intent(in) :: a, b
       integer(c_int64_t), intent(in) :: t0, t1, rate
-      write (*, '(A,A,A,I0,A,I0,A,F12.3)') 'timing ', label, ' iconf', a, '-', b, ' seconds ', &
-         real(t1 - t0, r8b)/real(rate, r8b)
+      write (*, '(A,1x,A,1x,A,I0,1x,A,I0,1x,A,F12.3,1x,A,I0,A,I0)') 'timing', trim(label), 'rank', 0, &
+         'iconf', a, 'seconds', real(t1 - t0, r8b)/real(rate, r8b), 'configs ', a, '-', b
+      flush (6)
+   end subroutine
+
+   ! progress.txt and the progress line of the reference (Write_progress,
+   ! J1J2_6tv.f90:499-529), for the batch whose first config is iconf.
+   subroutine write_progress(phase, iconf, current, total)
+      character(len=*), intent(in) :: phase
+      integer, intent(in) :: iconf, current, total
+      character(len=64) :: job_id
+      integer :: u
+      job_id = ''
+      call get_environment_variable('SLURM_JOB_ID', job_id)
+      open (newunit=u, file='progress.txt', status='replace')
+      write (u, '(A,1x,A)') 'job_id', trim(job_id)
+      write (u, '(A,1x,A)') 'sampler', trim(SAMPLERMODE)
+      write (u, '(A,1x,A)') 'phase', trim(phase)
+      write (u, '(A,1x,I0)') 'iconf', iconf
+      write (u, '(A,1x,I0)') 'nconf', NCONF
+      write (u, '(A,1x,I0)') 'ntasks', 1
+      write (u, '(A,1x,I0)') 'current', current
+      write (u, '(A,1x,I0)') 'total', total
+      close (u)
+      write (*, '(A,1x,A,1x,A,I0,1x,A,I0,1x,A,I0)') 'progress', trim(phase), 'iconf', iconf, &
+         'step', current, 'of', total
+      flush (6)
+   end subroutine
+
+   ! One PT phase of `total` cycles in chunks of max(1, total/10), with the
+   ! reference's progress reports (J1J2_6tv.f90:1267-1276, 1430-1531); the
+   ! chunks continue one run (the first begins the phase), so the result is
+   ! that of a single j1j2_run.
+   subroutine run_pt_phase(eng, phase, label, iconf, total)
+      type(c_ptr), intent(in) :: eng
+      integer(c_int), intent(in) :: phase
+      character(len=*), intent(in) :: label
+      integer, intent(in) :: iconf, total
+      integer :: every, done, chunk
+      every = max(1, total/10)
+      call write_progress(label, iconf, 0, total)
+      done = 0
+      chunk = min(every, total)
+      call j1j2_check(j1j2_run_cycles(eng, phase, int(chunk, i4b), 1_i4b), label)
+      done = chunk
+      if (done > 0) call write_progress(label, iconf, done, total)
+      do while (done < total)
+         chunk = min(every, total - done)
+         call j1j2_check(j1j2_run_cycles(eng, phase, int(chunk, i4b), 0_i4b), label)
+         done = done + chunk
+         call write_progress(label, iconf, done, total)
+      end do
    end subroutine
 
    ! ------------------------------------------------------------ PT
@@ -328,11 +394,11 @@ contains
       call write_phi_sites(fields(:, n))
 
       call system_clock(t0, rate)
-      call j1j2_check(j1j2_run(eng, J1J2_PHASE_EQUIL, -1_i4b), 'equilibration')
+      call run_pt_phase(eng, J1J2_PHASE_EQUIL, 'PT_EQUIL', first, NEQ)
       call system_clock(t1)
-      call j1j2_check(j1j2_run(eng, J1J2_PHASE_MEAS, -1_i4b), 'measurement')
-      call system_clock(t2)
       call print_timing('PT_EQUIL', first, first + n - 1, t0, t1, rate)
+      call run_pt_phase(eng, J1J2_PHASE_MEAS, 'PT_MEAS', first, NMESS)
+      call system_clock(t2)
       call print_timing('PT_MEAS', first, first + n - 1, t1, t2, rate)
 
       do k = 0, n - 1

@@ -169,6 +169,17 @@ def test_host_pt_matches_python_binding(tmp_path):
     assert nu.shape[0] == C * R * L ** 3
     np.testing.assert_array_equal(nu[:, 3].reshape(C, R, L ** 3), flips)
     assert os.path.exists(tmp_path / "locnem0001.dat")
+    # the phases ran in chunks of a tenth (3 cycles) with the reference's
+    # progress reports (Write_progress) and timing lines (Write_timing)
+    prog = dict(ln.split(" ", 1) for ln in open(str(tmp_path / "progress.txt")).read().splitlines())
+    assert prog["sampler"] == "PT" and prog["phase"] == "PT_MEAS" and prog["iconf"] == "3"
+    assert prog["nconf"] == "3" and prog["current"] == "60" and prog["total"] == "60"
+    steps = re.findall(r"^progress PT_EQUIL iconf1 step(\d+) of(\d+)$", r.stdout, re.M)
+    assert [int(s) for s, _ in steps] == list(range(0, 31, 3)) and {t for _, t in steps} == {"30"}
+    timing = re.findall(r"^timing (\S+) rank0 iconf(\d+) seconds +([0-9.]+) configs (\d+)-(\d+)$", r.stdout, re.M)
+    assert [(t[0], t[1], t[3], t[4]) for t in timing] == [
+        (lab, a, a, b) for a, b in (("1", "2"), ("3", "3"))
+        for lab in ("PT_EQUIL", "PT_MEAS", "COLLECT_DATA", "CONFIG_TOTAL")]
 
 
 @pytest.mark.gpu
