@@ -95,34 +95,44 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline():
-    """The unmodified reference (oracle/_ref/j1j2_cpu_l32: L=32, NREPLICA=16,
-    PTSWAPEVER=5, NEQ=100 cycles, one disorder config per MPI rank) on this
-    host's cores; su/s from its own 'timing PT_EQUIL' lines."""
-    exe = os.path.join(REPO, "oracle", "_ref", "j1j2_cpu_l32")
+def reference_rate(name, L_, nrep, cycles, ranks):
+    """Run the unmodified reference built from source (oracle/_ref/<name>:
+    PT, L_^3 sites, nrep replicas, PTSWAPEVER=5, `cycles` equilibration
+    cycles, one disorder config per MPI rank) on `ranks` host cores; su/s
+    from its own 'timing PT_EQUIL' lines, or None if it cannot run here."""
+    exe = os.path.join(REPO, "oracle", "_ref", name)
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
-    if os.path.exists(exe) and os.path.exists(mpiexec):
-        ranks = min(16, os.cpu_count() or 1)
-        tmp = tempfile.mkdtemp(prefix="j1j2cpu")
-        env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib:" + os.environ.get("LD_LIBRARY_PATH", ""))
-        try:
-            out = subprocess.run([mpiexec, "-n", str(ranks), exe], cwd=tmp, env=env, timeout=300,
-                                 capture_output=True, text=True)
-            secs = {}
-            for m in re.finditer(r"timing PT_EQUIL rank\s*(\d+) iconf\s*(\d+) seconds\s*([0-9.]+)", out.stdout):
-                secs.setdefault(int(m.group(1)), []).append(float(m.group(3)))
-            if secs:
-                per_cfg = float(L * L * L) * NREP * 100 * PTSWAPEVER
-                # ranks run concurrently: each rank's rate over its first config
-                rate = sum(per_cfg / v[0] for v in secs.values() if v[0] > 0)
-                return dict(value=rate, unit="site updates/s", cores=len(secs), kind="reference",
-                            sample=f"J1J2_6tv.f90 built from source (amdflang -O2, MPICH), L=32, "
-                                   f"NREPLICA=16, 100 PT cycles x 5 sweeps, one config per rank, "
-                                   f"{len(secs)} ranks; rate from its PT_EQUIL timing lines")
-        except (subprocess.SubprocessError, OSError):
-            pass
-        finally:
-            shutil.rmtree(tmp, ignore_errors=True)
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    tmp = tempfile.mkdtemp(prefix="j1j2cpu")
+    env = dict(os.environ, LD_LIBRARY_PATH="/opt/conda/lib:" + os.environ.get("LD_LIBRARY_PATH", ""))
+    try:
+        out = subprocess.run([mpiexec, "-n", str(ranks), exe], cwd=tmp, env=env, timeout=300,
+                             capture_output=True, text=True)
+        secs = {}
+        for m in re.finditer(r"timing PT_EQUIL rank\s*(\d+) iconf\s*(\d+) seconds\s*([0-9.]+)", out.stdout):
+            secs.setdefault(int(m.group(1)), []).append(float(m.group(3)))
+        if not secs:
+            return None
+        per_cfg = float(L_ ** 3) * nrep * cycles * PTSWAPEVER
+        # ranks run concurrently: each rank's rate over its first config
+        rate = sum(per_cfg / v[0] for v in secs.values() if v[0] > 0)
+        return dict(value=rate, unit="site updates/s", cores=len(secs), kind="reference",
+                    sample=f"J1J2_6tv.f90 built from source (amdflang -O2, MPICH), L={L_}, "
+                           f"NREPLICA={nrep}, {cycles} PT cycles x 5 sweeps, one config per rank, "
+                           f"{len(secs)} rank(s); rate from its PT_EQUIL timing lines")
+    except (subprocess.SubprocessError, OSError):
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_baseline():
+    """The unmodified reference at L=32, NREPLICA=16, 100 cycles, one config
+    per MPI rank on up to 16 host cores (else the oracle, single thread)."""
+    ref = reference_rate("j1j2_cpu_l32", L, NREP, 100, min(16, os.cpu_count() or 1))
+    if ref:
+        return ref
     # fall back to the oracle's restatement, single thread
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import numpy as np
@@ -213,7 +223,7 @@ WORKLOADS = {
 }
 
 
-def other_workload(J, name, steps, warmup, with_fp64=True):
+def other_workload(J, name, steps, warmup, with_fp64=True, with_cpu=True):
     desc, groups = WORKLOADS[name]
     fp32 = time_groups(J, groups, J.FP32_BATCHED, steps, warmup)
     n64 = max(steps // 10, 5)
@@ -233,6 +243,10 @@ def other_workload(J, name, steps, warmup, with_fp64=True):
         "fp64_plain": ({"value": fp64["rate"], "steps": n64, "fp32_over_fp64": fp32["rate"] / fp64["rate"]}
                        if fp64 else None),
         "flip_mismatch_rate": {"fp32_vs_fp64": rate, "proposals": n},
+        # L = 16, one config: the reference runs it on one core (configs are
+        # its unit of parallelism), 1000 cycles (~10 s)
+        "cpu_baseline": (reference_rate("j1j2_cpu_l16", 16, 16, 1000, 1)
+                         if name == "c2" and with_cpu else None),
     }
 
 
@@ -266,7 +280,11 @@ def main():
         del src, dst
 
     if args.workload != "c3":
-        print(json.dumps(other_workload(J, args.workload, args.steps, args.warmup, not args.no_fp64)), flush=True)
+        out = other_workload(J, args.workload, args.steps, args.warmup, not args.no_fp64, not args.no_cpu)
+        cpu = out["cpu_baseline"]
+        if cpu and out["fp64_plain"]:
+            out["fp64_plain"]["over_cpu_reference"] = out["fp64_plain"]["value"] / cpu["value"]
+        print(json.dumps(out), flush=True)
         return
 
     fp32 = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=True)

@@ -33,6 +33,7 @@ pt_l8_stat    L=8 H=8 SAMPLERMODE='PT' NREPLICA=4 TMIN=8.0D0 TMAX=11.5D0 START='
 local_l8      L=8 H=8 SAMPLERMODE='LOCAL' TMIN=10.0D0 TMAX=10.0D0 START='HOT' NEQ=1000 NMESS=1000 NCONF=1 IRINIT=1 WRITEBINDTIME=.false.
 local_frozen  L=8 H=8 SAMPLERMODE='LOCAL' TMIN=0.01D0 TMAX=0.01D0 START='FER' phi=0.0D0 NEQ=10 NMESS=200 NCONF=1 IRINIT=1 WRITEBINDTIME=.false.
 cpu_l32       L=32 H=32 SAMPLERMODE='PT' NREPLICA=16 TMIN=8.0D0 TMAX=11.5D0 START='STR' STRAXIS='X' NEQ=100 NMESS=0 NCONF=16 IRINIT=1 WRITEBINDTIME=.false. WRITELOCNEM=.false. WRITEHIST=.false.
+cpu_l16       L=16 H=16 SAMPLERMODE='PT' NREPLICA=16 TMIN=8.0D0 TMAX=11.5D0 START='STR' STRAXIS='X' NEQ=1000 NMESS=0 NCONF=1 IRINIT=1 WRITEBINDTIME=.false. WRITELOCNEM=.false. WRITEHIST=.false.
 EOF
 }
 
