@@ -10,6 +10,10 @@ Workload (one step = one PT cycle over the whole batch, inputs resident in HBM):
 Printed as one JSON line: the FP32 batched form's rate (value), the plain FP64
 form's rate on the same workload, the sweep kernel's roofline figures and the
 reference's CPU rate on a bounded sample of the same workload.
+
+Before the timed region: the autotuner's cycles, --warmup equilibration
+cycles and 17 measuring cycles that build both measuring hipGraphs, so the
+timed steps are steady-state replays.
 """
 import argparse
 import json
@@ -27,6 +31,8 @@ sys.path.insert(0, os.path.join(REPO, "j1j2-public_amd"))
 L, NREP, NCONF, PTSWAPEVER = 32, 16, 64, 5
 HBM_PEAK = 8.0e12                  # MI355X HBM3E, bytes/s
 TUNE_CYCLES = 40                   # the autotuner times its candidates on at most 36 cycles of one call
+GRAPH_WARM = 17                    # one 16-cycle graph launch and one single-cycle graph launch
+FP64_MIN_STEPS = 20
 SEED = 20261203
 
 
@@ -44,16 +50,23 @@ def build_batch(J, precision, flags=0, nconf=NCONF):
     return eng
 
 
-def time_form(J, precision, steps, warmup, kernel_timing, nconf=NCONF):
-    """Timed region: `steps` MEAS cycles replayed as hipGraphs.  With
-    kernel_timing, a second pass of the same length brackets every sweep
-    launch with HIP events on the engine's stream (launched eagerly) for the
-    sweep kernel's average duration."""
-    eng = build_batch(J, precision, nconf=nconf)
-    eng.run(J.PHASE_EQUIL, TUNE_CYCLES)      # the autotuner's cycles, before any warmup or timed step
+def warm(J, eng, warmup):
+    """Untimed lead-in: the autotuner's cycles, `warmup` equilibration cycles,
+    then GRAPH_WARM measuring cycles, which capture and instantiate both
+    measuring hipGraphs (16 cycles and one cycle) before the timed region."""
+    eng.run(J.PHASE_EQUIL, TUNE_CYCLES)
     eng.run(J.PHASE_EQUIL, warmup)
-    eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
+    eng.run_cycles(J.PHASE_MEAS, GRAPH_WARM, begin=True)
     eng.synchronize()
+
+
+def time_form(J, precision, steps, warmup, kernel_timing, nconf=NCONF):
+    """Timed region: `steps` MEAS cycles replayed as hipGraphs (both graphs
+    built during the warm-up).  With kernel_timing, a second pass of the same
+    length brackets every sweep launch with HIP events on the engine's stream
+    (launched eagerly) for the sweep kernel's average duration."""
+    eng = build_batch(J, precision, nconf=nconf)
+    warm(J, eng, warmup)
     t0 = time.perf_counter()
     eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
     eng.synchronize()
@@ -85,14 +98,48 @@ def mismatch_rate(J, cycles):
     return mis / max(tot, 1), tot
 
 
+PMC_SUMMARY = os.path.join("profiles", "k_sweep_resident_l32_16x64_pmc.md")
+
+
 def traffic_from_profiles():
     """HBM bytes per sweep-kernel launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(REPO, "profiles", "sweep_resident_l32_pmc.txt")
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    and the block size of the kernel variant those counters were read on."""
+    path = os.path.join(REPO, PMC_SUMMARY)
     if not os.path.exists(path):
-        return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
-    return float(m.group(1)) if m else None
+        return None, None
+    text = open(path).read()
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
+    nt = re.search(r"^block_threads:\s*(\d+)", text, re.M)
+    return (float(m.group(1)) if m else None), (int(nt.group(1)) if nt else None)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def physical_cores():
+    """Distinct (physical id, core id) pairs of /proc/cpuinfo."""
+    cores, phys = set(), None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":", 1)[1].strip()
+            elif line.startswith("core id"):
+                cores.add((phys, line.split(":", 1)[1].strip()))
+    except OSError:
+        pass
+    return len(cores) or (os.cpu_count() or 1)
+
+
+# host cores this one-GPU job may use (the GPU host's share for one GPU)
+CPU_SHARE = 16
 
 
 def reference_rate(name, L_, nrep, cycles, ranks):
@@ -120,7 +167,8 @@ def reference_rate(name, L_, nrep, cycles, ranks):
         return dict(value=rate, unit="site updates/s", cores=len(secs), kind="reference",
                     sample=f"J1J2_6tv.f90 built from source (amdflang -O2, MPICH), L={L_}, "
                            f"NREPLICA={nrep}, {cycles} PT cycles x 5 sweeps, one config per rank, "
-                           f"{len(secs)} rank(s); rate from its PT_EQUIL timing lines")
+                           f"{len(secs)} rank(s) on {cpu_model()}; rate = sum over ranks of each "
+                           f"rank's PT_EQUIL timing line")
     except (subprocess.SubprocessError, OSError):
         return None
     finally:
@@ -129,9 +177,13 @@ def reference_rate(name, L_, nrep, cycles, ranks):
 
 def cpu_baseline():
     """The unmodified reference at L=32, NREPLICA=16, 100 cycles, one config
-    per MPI rank on up to 16 host cores (else the oracle, single thread)."""
-    ref = reference_rate("j1j2_cpu_l32", L, NREP, 100, min(16, os.cpu_count() or 1))
+    per MPI rank, P = min(NCONF, physical cores, the job's host-core share)
+    ranks (else the oracle, single thread)."""
+    ranks = min(NCONF, physical_cores(), CPU_SHARE, os.cpu_count() or 1)
+    ref = reference_rate("j1j2_cpu_l32", L, NREP, 100, ranks)
     if ref:
+        ref["rank_rule"] = (f"P = min(NCONF = {NCONF}, physical cores = {physical_cores()}, "
+                            f"host-core share of a one-GPU job = {CPU_SHARE}) = {ranks}")
         return ref
     # fall back to the oracle's restatement, single thread
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -180,10 +232,7 @@ def build_groups(J, groups, precision, flags=0, seed=SEED):
 
 def time_groups(J, groups, precision, steps, warmup, flags=0):
     eng = build_groups(J, groups, precision, flags)
-    eng.run(J.PHASE_EQUIL, TUNE_CYCLES)      # the autotuner's cycles, before any warmup or timed step
-    eng.run(J.PHASE_EQUIL, warmup)
-    eng.run_cycles(J.PHASE_MEAS, 0, begin=True)
-    eng.synchronize()
+    warm(J, eng, warmup)
     t0 = time.perf_counter()
     eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
     eng.synchronize()
@@ -226,7 +275,7 @@ WORKLOADS = {
 def other_workload(J, name, steps, warmup, with_fp64=True, with_cpu=True):
     desc, groups = WORKLOADS[name]
     fp32 = time_groups(J, groups, J.FP32_BATCHED, steps, warmup)
-    n64 = max(steps // 10, 5)
+    n64 = max(steps // 10, FP64_MIN_STEPS)
     fp64 = time_groups(J, groups, J.FP64_PLAIN, n64, max(warmup // 10, 2)) if with_fp64 else None
     su_launch = fp32["su"] / max(fp32["launches"], 1)
     avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)
@@ -262,7 +311,11 @@ def main():
                     help="c3 = the headline L=32 batch; the others are reported beside it")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run 5 plain device copies of this many bytes (PMC counter calibration)")
+    ap.add_argument("--block-threads", type=int, default=0,
+                    help="fix the resident sweep's threads per block (no autotuning), e.g. for counter runs")
     args = ap.parse_args()
+    if args.block_threads:
+        os.environ["J1J2_RES_THREADS"] = str(args.block_threads)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: measures one GPU only; the path would shard disorder configs across GPUs "
               "(one process per GPU, each its own engine over its share of the configs, no exchange "
@@ -309,22 +362,37 @@ def main():
     su_launch = fp32["su"] / max(fp32["launches"], 1)
     avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)
     achieved = su_launch * fp32["bytes_per"] / avg_s
-    traffic = traffic_from_profiles()
+    traffic, traffic_nt = traffic_from_profiles()
+    timed_nt = re.search(r"nt(\d+)", fp32["plan"])
     res["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK, "traffic": traffic,
                        "kernel": "k_sweep_resident", "avg_launch_ms": avg_s * 1e3,
                        "site_updates_per_launch": su_launch,
-                       "algorithmic_bytes_per_update": fp32["bytes_per"]}
+                       "algorithmic_bytes_per_update": fp32["bytes_per"],
+                       # achieved/frac are the algorithmic-byte equivalent (6 B per site
+                       # update over the launch time); the bytes that cross HBM are the
+                       # counters' traffic, read on the variant named here
+                       "achieved_basis": "algorithmic bytes (6 B per site update) / average launch time",
+                       "traffic_source": PMC_SUMMARY,
+                       "traffic_block_threads": traffic_nt,
+                       "timed_block_threads": int(timed_nt.group(1)) if timed_nt else None,
+                       "measured_hbm_GBps": (traffic / avg_s / 1e9) if traffic else None,
+                       "limiter": "VALU issue (the lattice stays in LDS for the launch's 5 sweeps)"}
     if not args.no_fp64:
-        n64 = args.fp64_steps or max(args.steps // 10, 5)
+        n64 = args.fp64_steps or max(args.steps // 10, FP64_MIN_STEPS)
         fp64 = time_form(J, J.FP64_PLAIN, n64, max(args.warmup // 10, 2), kernel_timing=False)
         res["fp64_plain"] = {"value": fp64["rate"], "steps": n64, "ms_per_step": 1e3 * fp64["seconds"] / n64,
-                             "fp32_over_fp64": fp32["rate"] / fp64["rate"]}
+                             "fp32_over_fp64": fp32["rate"] / fp64["rate"],
+                             "reading": "same batching as the FP32 form: one engine over the 64 configs x "
+                                        "16 replicas, FP64 field in the reference's order, global-memory "
+                                        "colour passes"}
         # the same FP64 form run the reference's way, one disorder config at a
         # time (one engine of 16 replicas)
         one = time_form(J, J.FP64_PLAIN, 4 * n64, max(args.warmup // 10, 2), kernel_timing=False, nconf=1)
         res["fp64_plain"]["one_config_at_a_time"] = {"value": one["rate"], "steps": 4 * n64,
-                                                     "fp32_over_fp64": fp32["rate"] / one["rate"]}
+                                                     "fp32_over_fp64": fp32["rate"] / one["rate"],
+                                                     "reading": "configs one at a time as the reference "
+                                                                "loops over them (an engine of 16 replicas)"}
         rate, n = mismatch_rate(J, 4)
         res["flip_mismatch_rate"] = {"fp32_vs_fp64": rate, "proposals": n}
     if not args.no_cpu:

@@ -110,6 +110,7 @@ struct Engine {
     Plan plan;                                 // resident launches of a cycle
     std::vector<OrderSeg> group_seg;           // work-list segment of each resident group
     OrderSeg all_seg;                          // of all lattices (when all are resident)
+    int32_t order_len = 0;                     // entries of the device work list (st.order)
     bool tuned = false, autotune = true;
     std::string tune_key;
     std::vector<hipStream_t> side;             // one per concurrent launch
@@ -374,6 +375,19 @@ struct Probe {
     hipEvent_t phase_end = nullptr;
 };
 
+// Every resident launch must read its blocks' lattices inside the device work
+// list: a segment past its end would send blocks to arbitrary lattices.
+int check_plan(const Engine &E, const Plan &P)
+{
+    for (const ResLaunch &r : P.launches)
+        if (r.ord0 < 0 || r.nord < 0 || (int64_t)r.ord0 + r.nord > E.order_len || (r.nord & 7))
+            return fail(J1J2_EINVAL, "internal error: a resident launch's work-list segment [" +
+                                         std::to_string(r.ord0) + ", +" + std::to_string(r.nord) +
+                                         ") lies outside the work list of " + std::to_string(E.order_len) +
+                                         " entries");
+    return 0;
+}
+
 int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe = nullptr)
 {
     // mode bit 0: measure; bit 1: values only (no sweeps, no swaps)
@@ -381,6 +395,7 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
     int rc;
     if (!values_only) {
         const Plan &P = plan ? *plan : E.plan;
+        if ((rc = check_plan(E, P))) return rc;
         if (P.concurrent && P.launches.size() > 1) {
             // fork: every launch on its own branch, join before the swap step
             auto fn = [](Engine &e, const void *a) {
@@ -489,9 +504,19 @@ std::mutex &tune_mutex()
     return m;
 }
 
-std::map<std::string, Plan> &tune_cache()
+// What the autotuner chose for an engine shape.  Only the choices are cached,
+// never work-list offsets: every engine rebuilds its launches from its own
+// work-list segments (plan_from_choice), so an engine whose configs split the
+// same lattices differently still launches exactly its own lists.
+struct TuneChoice {
+    std::vector<int32_t> pick;   // per resident group, in group order: threads per block (byte layout) or slabs (bits)
+    bool concurrent = false;     // the per-group launches on parallel graph branches
+    int32_t all_nt = 0;          // > 0: one launch over all lattices with this many threads per block
+};
+
+std::map<std::string, TuneChoice> &tune_cache()
 {
-    static std::map<std::string, Plan> m;
+    static std::map<std::string, TuneChoice> m;
     return m;
 }
 
@@ -513,6 +538,28 @@ ResLaunch all_launch(const Engine &E, int32_t nt)
     ResLaunch r{E.all_seg.ord0, E.all_seg.nord, E.uniform_L, E.max_n, nt, E.st.nlat};
     r.H = E.uniform_L ? E.uniform_H : (E.all_cubic ? -1 : 0);
     return r;
+}
+
+Plan plan_from_choice(const Engine &E, const TuneChoice &t)
+{
+    Plan P;
+    if (t.all_nt > 0 && E.all_byte) {
+        P.launches.push_back(all_launch(E, t.all_nt));
+        return P;
+    }
+    size_t i = 0;
+    for (int g = 0; g < (int)E.groups.size(); ++g) {
+        const Group &G = E.groups[g];
+        if (!G.resident) continue;
+        int32_t v = i < t.pick.size() ? t.pick[i] : (G.kind == 2 ? 1 : E.res_nt);
+        ++i;
+        if (G.kind == 2 && v > 1 && !bits_slab_fits(G.p.L, G.p.H, v, G.slab_seg[v].nord, E.shadow)) v = 1;
+        if (G.kind == 1 && !resident_nt_valid(G.p.L, v)) v = 256;
+        P.launches.push_back(group_launch(E, g, v, v));
+    }
+    P.concurrent = t.concurrent && P.launches.size() > 1 &&
+                   std::none_of(P.launches.begin(), P.launches.end(), [](const ResLaunch &r) { return r.slabs > 1; });
+    return P;
 }
 
 Plan default_plan(const Engine &E)
@@ -627,9 +674,11 @@ std::string tune_key(const Engine &E)
     hipDeviceProp_t prop{};
     (void)hipGetDeviceProperties(&prop, E.device);
     std::string k = std::string(prop.gcnArchName) + (E.shadow ? " shadow" : "");
+    // the configs x replicas split fixes the work lists' shapes (their XCD
+    // balance and padding), so it is part of the shape a choice was timed on
     for (const Group &G : E.groups)
-        k += " " + std::to_string(G.p.L) + "x" + std::to_string(G.p.H) + ":" + std::to_string(G.nlat) +
-             (G.resident ? "r" : "g");
+        k += " " + std::to_string(G.p.L) + "x" + std::to_string(G.p.H) + ":" + std::to_string(G.p.nconf) + "x" +
+             std::to_string(G.p.nreplica) + (G.resident ? "r" : "g");
     return k;
 }
 
@@ -653,7 +702,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         std::lock_guard<std::mutex> lk(tune_mutex());
         auto it = tune_cache().find(E.tune_key);
         if (it != tune_cache().end()) {
-            E.plan = it->second;
+            E.plan = plan_from_choice(E, it->second);
             E.tuned = true;
             drop_graphs(E);
             return ensure_side(E, E.plan.concurrent ? E.plan.launches.size() : 0);
@@ -706,6 +755,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
                                  ms[i]);
         }
     Plan per;
+    TuneChoice per_choice;
     float t_seq = 0.0f;
     // a candidate within TUNE_TIE of the fastest counts as a tie and the
     // earlier one (fewer threads) is kept, so that the choice is stable
@@ -714,25 +764,35 @@ int tune(Engine &E, int measure, int ncycles, int &done)
         size_t b = 0;
         while (best[i][b] > tmin * TUNE_TIE) ++b;
         per.launches.push_back(group_launch(E, R[i], cand[i][b], cand[i][b]));
+        per_choice.pick.push_back(cand[i][b]);
         t_seq += best[i][b];
     }
     Plan choice = per;
+    TuneChoice tc = per_choice;
     float t_best = t_seq;
     if (multi) {
         // 2. the same launches on parallel branches, and one launch over all
         std::vector<Plan> alt;
+        std::vector<TuneChoice> alt_choice;
         Plan conc = per;
         conc.concurrent = true;
         // slab launches need all their blocks resident at once: not beside others
-        if (std::none_of(per.launches.begin(), per.launches.end(), [](const ResLaunch &r) { return r.slabs > 1; }))
+        if (std::none_of(per.launches.begin(), per.launches.end(), [](const ResLaunch &r) { return r.slabs > 1; })) {
             alt.push_back(conc);
+            alt_choice.push_back(per_choice);
+            alt_choice.back().concurrent = true;
+        }
         if (E.all_byte)
             for (int32_t nt : ALL_NT) {
                 Plan one;
                 one.launches.push_back(all_launch(E, nt));
                 alt.push_back(one);
+                TuneChoice c;
+                c.all_nt = nt;
+                alt_choice.push_back(c);
             }
-        for (const Plan &P : alt) {
+        for (size_t a = 0; a < alt.size(); ++a) {
+            const Plan &P = alt[a];
             float tmin = 1e30f;
             for (int rep = 0; rep < TUNE_REPS; ++rep) {
                 std::vector<float> ms;
@@ -745,6 +805,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
             if (tmin * TUNE_TIE < t_best) {
                 t_best = tmin;
                 choice = P;
+                tc = alt_choice[a];
             }
         }
     }
@@ -753,9 +814,21 @@ int tune(Engine &E, int measure, int ncycles, int &done)
     drop_graphs(E);
     {
         std::lock_guard<std::mutex> lk(tune_mutex());
-        tune_cache()[E.tune_key] = choice;
+        tune_cache()[E.tune_key] = tc;
     }
     return ensure_side(E, choice.concurrent ? choice.launches.size() : 0);
+}
+
+// FP32 rule: |y| = |b2 h| must stay far inside the int range of rint(y)
+// (threshold32 has no clamp); 2^24 leaves every threshold exactly 0 or inf.
+int check_temperatures(const Engine &E)
+{
+    if (E.common.precision == J1J2_FP32_BATCHED)
+        for (int c = 0; c < (int)E.confs.size(); ++c)
+            for (double t : E.temps[c])
+                if (2.0 * LOG2E / t * E.hbound[c] >= 0x1p24)
+                    return fail(J1J2_EINVAL, "temperature too low for the FP32 form (|beta dE| >= 2^24 ln 2)");
+    return 0;
 }
 
 int run_cycles(Engine &E, int phase, int ncycles, int begin)
@@ -763,13 +836,7 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
     if (phase != J1J2_PHASE_EQUIL && phase != J1J2_PHASE_MEAS) return fail(J1J2_EINVAL, "unknown phase");
     const int measure = phase == J1J2_PHASE_MEAS ? 1 : 0;
     int rc;
-    // FP32 rule: |y| = |b2 h| must stay far inside the int range of rint(y)
-    // (threshold32 has no clamp); 2^24 leaves every threshold exactly 0 or inf.
-    if (E.common.precision == J1J2_FP32_BATCHED)
-        for (int c = 0; c < (int)E.confs.size(); ++c)
-            for (double t : E.temps[c])
-                if (2.0 * LOG2E / t * E.hbound[c] >= 0x1p24)
-                    return fail(J1J2_EINVAL, "temperature too low for the FP32 form (|beta dE| >= 2^24 ln 2)");
+    if ((rc = check_temperatures(E))) return rc;
     if (begin) {
         HIPCHK(launch_begin_phase(E.st, E.stream));
         if (measure)
@@ -1041,6 +1108,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
                 !bits_slab_fits(G.p.L, G.p.H, E->bit_slabs, G.slab_seg[E->bit_slabs].nord, E->shadow))
                 E->bit_slabs = 1;
         int32_t *dorder;
+        E->order_len = (int32_t)order.size();
         if ((rc = E->alloc(dorder, (int64_t)order.size()))) return rc;
         if (!order.empty() && (rc = h2d(*E, dorder, order))) return rc;
         st.order = dorder;

@@ -81,3 +81,35 @@ def test_autotuned_bit_layout_is_bit_identical():
     assert "default" in plan_d and " s" not in plan_d
     eng_d.close()
     same(st_t, st_d)
+
+
+@pytest.mark.parametrize("splits", [
+    [[(32, 16, 1)], [(32, 8, 2)]],                                   # 16 lattices of L = 32 either way
+    [[(16, 8, 2), (32, 16, 1)], [(16, 16, 1), (32, 8, 2)]],          # two sizes, 16 lattices each
+])
+def test_cached_plan_follows_each_engines_work_list(splits):
+    """The autotuner's choice is cached per process.  Engines over the same
+    lattices split differently into configs x replicas have work lists of
+    different shapes; each engine must launch its own list whether it tuned
+    or took a cached choice.  Every engine (tuned, then re-created with the
+    cache filled, for each split in turn) equals an untuned run bit for bit,
+    and its last config equals the oracle."""
+    for rnd in range(2):
+        for split in splits:
+            groups, iconf = [], 1
+            for L, R, C in split:
+                groups.append(J.make_params(L, L, nreplica=R, nconf=C, iconf0=iconf, seed=SEED))
+                iconf += C
+            say(f"case cached plan: split {split}, round {rnd}")
+            eng_t, inputs, plan_t, st_t = run(groups, "hot", True, CYCLES)
+            say(f"  plan: {plan_t}")
+            assert "autotuned" in plan_t
+            gg, ic, p, s = inputs[-1]
+            ora = oracle_for(gg, ic, p, s)
+            ora.run(CYCLES, False)
+            ora.run(CYCLES, True)
+            compare_state(eng_t, eng_t.nconf - 1, ora, J.ladder(gg))
+            eng_t.close()
+            eng_d, _, plan_d, st_d = run(groups, "hot", False, CYCLES)
+            eng_d.close()
+            same(st_t, st_d)
