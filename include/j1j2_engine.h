@@ -146,6 +146,23 @@ int  j1j2_set_temperatures(void *handle, int32_t conf, const double *temps, int3
 int  j1j2_run(void *handle, int32_t phase, int32_t ncycles);
 int  j1j2_run_cycles(void *handle, int32_t phase, int32_t ncycles, int32_t begin);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory once an engine exists: code under their own HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Enqueues the same
+ * cycles as j1j2_run_cycles (bit-identical results) on `stream` (a
+ * hipStream_t; NULL = the engine's own stream) and returns without waiting:
+ * no device or host allocation, no graph build, no autotuning, no events, no
+ * synchronisation, accumulators cleared by fill kernels.  The partial
+ * observables a cycle's sweep hands to its swap step live in one static
+ * device array per GPU of J1J2_NOALLOC_MAX_LATTICES x 8 int64 (1 MiB): an
+ * engine with more lattices (configs x replicas) is rejected with J1J2_EINVAL.
+ * All calls on one GPU share that array, so calls that overlap on one GPU
+ * (other streams, other engines) race.  Wait with j1j2_synchronize (or on
+ * `stream`, before any other call on the handle); j1j2_synchronize also
+ * reports a bounded z-slab wait that ran out. */
+#define J1J2_NOALLOC_MAX_LATTICES 16384
+int  j1j2_run_cycles_noalloc(void *handle, int32_t phase, int32_t ncycles, int32_t begin, void *stream);
+
 /* Results of one config after MEAS, packed as Collect_data does
  * (J1J2_6tv.f90:2937-2999).  Any pointer may be NULL.
  *   basefields : 49*nreplica, field-major, slot fastest (BASEFIELDS layout)

@@ -874,6 +874,85 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
     return 0;
 }
 
+// ---------------------------------------------------------------- no-alloc cycles
+// reset_measurement for every config, by fill kernels on stream q
+int reset_measurement_fill(const Engine &E, hipStream_t q)
+{
+    const DeviceState &st = E.st;
+    const int64_t nconf = (int64_t)E.confs.size();
+    uint64_t dmax;
+    const double big = 1.7976931348623157e308;
+    std::memcpy(&dmax, &big, sizeof(dmax));
+    HIPCHK(launch_fill_u64(st.sums, E.nslots * NSUMS, 0, q));
+    HIPCHK(launch_fill_u64(st.best, E.nslots, dmax, q));
+    HIPCHK(launch_fill_u64(st.final_en, E.nslots, 0, q));
+    HIPCHK(launch_fill_u64(st.hist, E.nhist, 0, q));
+    HIPCHK(launch_fill_u64(st.hmin, E.nslots, 0, q));
+    HIPCHK(launch_fill_u64(st.hmax, E.nslots, 0, q));
+    HIPCHK(launch_fill_u64(st.hbin, E.nslots, 0, q));
+    HIPCHK(launch_fill_u8(st.vcold, E.nslots, 0, q));
+    HIPCHK(launch_fill_u8(st.vhot, E.nslots, 0, q));
+    HIPCHK(launch_fill_u64(st.nmeasure, nconf, 0, q));
+    HIPCHK(launch_fill_u8(st.locspin, E.nsites * (int64_t)sizeof(int32_t), 0, q));
+    HIPCHK(launch_fill_u8(st.locnem, E.nsites * (int64_t)sizeof(int32_t), 0, q));
+    HIPCHK(launch_fill_u64(st.locsamples, nconf, 0, q));
+    HIPCHK(launch_fill_u64(st.series_len, nconf, 0, q));
+    return 0;
+}
+
+// One cycle, eagerly on stream q with device state st: the plan's resident
+// launches one after another (a cycle's result does not depend on the plan),
+// the global-memory passes, the swap step.  No events, no side streams.
+int enqueue_cycle_noalloc(const Engine &E, const DeviceState &st, hipStream_t q, bool measure)
+{
+    int rc;
+    if ((rc = check_plan(E, E.plan))) return rc;
+    for (const ResLaunch &r : E.plan.launches)
+        HIPCHK(r.kind == 2 ? launch_sweep_bits(st, r.ord0, r.nord, r.L, r.H, r.slabs, E.shadow, q)
+                           : launch_sweep_resident(st, r.ord0, r.nord, r.max_n, r.L, r.H, r.nt, E.shadow, q));
+    for (const Group &G : E.groups) {
+        if (G.resident) continue;
+        for (int k = 0; k < E.common.ptswapever; ++k)
+            for (int c = 0; c < 4; ++c)
+                HIPCHK(E.common.precision == J1J2_FP64_PLAIN
+                           ? launch_sweep_plain64(st, G.lat0, G.nlat, G.p.L, G.p.H, k, c, q)
+                           : launch_sweep_global(st, G.lat0, G.nlat, G.p.L, G.p.H, k, c, E.shadow, q));
+        HIPCHK(launch_observables(st, G.lat0, G.nlat, q));
+    }
+    HIPCHK(launch_pt_step(st, measure ? 1 : 0, q));
+    if (measure && E.locnem && !st.fuse_locnem) HIPCHK(launch_locnem(st, E.max_n, 0, q));
+    return 0;
+}
+
+int run_cycles_noalloc(Engine &E, int phase, int ncycles, int begin, hipStream_t q)
+{
+    if (phase != J1J2_PHASE_EQUIL && phase != J1J2_PHASE_MEAS) return fail(J1J2_EINVAL, "unknown phase");
+    if (E.st.nlat > NOALLOC_MAX_LATTICES)
+        return fail(J1J2_EINVAL, "j1j2_run_cycles_noalloc: the engine has " + std::to_string(E.st.nlat) +
+                                     " lattices; its static scratch holds at most " +
+                                     std::to_string(NOALLOC_MAX_LATTICES));
+    int64_t *scr = noalloc_scratch(E.device);
+    if (!scr) return fail(J1J2_ESTATE, "j1j2_run_cycles_noalloc: scratch address not set up for this device");
+    const bool measure = phase == J1J2_PHASE_MEAS;
+    int rc;
+    if ((rc = check_temperatures(E))) return rc;
+    DeviceState st = E.st;
+    st.obs = scr;
+    if (begin) {
+        HIPCHK(launch_begin_phase(st, q));
+        if (measure && (rc = reset_measurement_fill(E, q))) return rc;
+    }
+    for (int i = 0; i < ncycles; ++i)
+        if ((rc = enqueue_cycle_noalloc(E, st, q, measure))) return rc;
+    E.cycles_done += (uint64_t)ncycles;
+    if (st.fuse_locnem && ncycles > 0) {
+        HIPCHK(launch_locnem(st, E.max_n, 1, q));
+        HIPCHK(launch_fill_u8(st.locnem_pending, (int64_t)E.confs.size() * (int64_t)sizeof(int32_t), 0, q));
+    }
+    if (ncycles > 0) E.observed = true;
+    return 0;
+}
+
 // Observables of the current state without advancing it (NMESS <= 0 branch
 // of Pt_measurement_loop, J1J2_6tv.f90:1414-1424).
 int refresh_values(Engine &E)
@@ -930,6 +1009,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(J1J2_EINVAL, "device index out of range");
     HIPCHK(hipSetDevice(device));
+    HIPCHK(noalloc_setup(device));    // address of the no-alloc scratch on this device, once
 
     Engine *E = new Engine();
     std::unique_ptr<Engine> guard(E);
@@ -1285,6 +1365,15 @@ int j1j2_run_cycles(void *handle, int32_t phase, int32_t ncycles, int32_t begin)
     return run_cycles(*E, phase, ncycles, begin);
 }
 
+int j1j2_run_cycles_noalloc(void *handle, int32_t phase, int32_t ncycles, int32_t begin, void *stream)
+{
+    Engine *E = as_engine(handle);
+    if (!E) return fail(J1J2_EINVAL, "null engine handle");
+    if (ncycles < 0) return fail(J1J2_EINVAL, "ncycles must be >= 0");
+    HIPCHK(hipSetDevice(E->device));
+    return run_cycles_noalloc(*E, phase, ncycles, begin, stream ? static_cast<hipStream_t>(stream) : E->stream);
+}
+
 int j1j2_run(void *handle, int32_t phase, int32_t ncycles)
 {
     Engine *E = as_engine(handle);
@@ -1564,6 +1653,12 @@ int j1j2_synchronize(void *handle)
     if (!E) return fail(J1J2_EINVAL, "null engine handle");
     HIPCHK(hipSetDevice(E->device));
     HIPCHK(hipStreamSynchronize(E->stream));
+    if (E->st.slab_err) {
+        int32_t err = 0;
+        HIPCHK(hipMemcpy(&err, E->st.slab_err, sizeof(err), hipMemcpyDeviceToHost));
+        if (err)
+            return fail(J1J2_EDEVICE, "bit-layout slabs: a wait for a neighbouring slab ran out (slabs not resident together)");
+    }
     return 0;
 }
 

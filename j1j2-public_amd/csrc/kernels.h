@@ -125,4 +125,15 @@ hipError_t launch_locnem(const DeviceState &st, int32_t max_n, int32_t only_pend
 // swap alternation restarts with the odd pairs).
 hipError_t launch_begin_phase(const DeviceState &st, hipStream_t s);
 
+// No-alloc cycles (noalloc.hip): the per-lattice partial observables of a
+// cycle in a static device array of NOALLOC_MAX_LATTICES x OBS_N int64 per GPU.
+// noalloc_setup looks its address up once per device (with that device
+// current); noalloc_scratch returns it (nullptr before the set-up).
+constexpr int32_t NOALLOC_MAX_LATTICES = 16384;
+hipError_t noalloc_setup(int device);
+int64_t *noalloc_scratch(int device);
+// n 64-bit words (resp. bytes) at p set to v, by a kernel (capturable).
+hipError_t launch_fill_u64(void *p, int64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_fill_u8(void *p, int64_t n, uint8_t v, hipStream_t s);
+
 }  // namespace j1j2

@@ -28,7 +28,9 @@ EXPORTS = [
     "j1j2_run_cycles", "j1j2_fetch", "j1j2_get_spins", "j1j2_get_maps", "j1j2_get_observables",
     "j1j2_get_sums", "j1j2_get_series", "j1j2_get_flips", "j1j2_get_mismatch", "j1j2_set_timing",
     "j1j2_get_timing", "j1j2_sweep_path", "j1j2_launch_plan", "j1j2_synchronize",
+    "j1j2_run_cycles_noalloc",
 ]
+NOALLOC_MAX_LATTICES = 16384
 
 
 class Params(ctypes.Structure):
@@ -95,6 +97,7 @@ def _declare(L):
         "j1j2_sweep_path": [P, P, P],
         "j1j2_launch_plan": [P, ctypes.c_char_p, i32],
         "j1j2_synchronize": [P],
+        "j1j2_run_cycles_noalloc": [P, i32, i32, i32, P],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -199,6 +202,10 @@ class Engine:
 
     def run_cycles(self, phase, ncycles, begin=False):
         check(lib().j1j2_run_cycles(self._h, phase, ncycles, 1 if begin else 0))
+
+    def run_cycles_noalloc(self, phase, ncycles, begin=False, stream=None):
+        """Enqueue without allocating or waiting; synchronize() before reading."""
+        check(lib().j1j2_run_cycles_noalloc(self._h, phase, ncycles, 1 if begin else 0, stream))
 
     def spins(self, conf):
         L, H, R = self.shape(conf)
