@@ -142,7 +142,10 @@ int  j1j2_set_temperatures(void *handle, int32_t conf, const double *temps, int3
  * PT cycle = PTSWAPEVER sweeps of every replica at its slot's temperature,
  * then one pass of adjacent swaps alternating odd/even pairs (J1J2_6tv.f90:
  * 1282-1338); MEAS cycles also accumulate the per-slot moments (J1J2_6tv.f90:
- * 1437-1524).  The swap alternation restarts at each call with begin != 0. */
+ * 1437-1524).  The swap alternation restarts at each call with begin != 0.
+ * j1j2_run of a measuring phase with no cycles (NMESS = 0) takes one
+ * local-nematic sample of the current state, as the reference's NMESS <= 0
+ * branches do (J1J2_6tv.f90:1414-1424, 2157-2160). */
 int  j1j2_run(void *handle, int32_t phase, int32_t ncycles);
 int  j1j2_run_cycles(void *handle, int32_t phase, int32_t ncycles, int32_t begin);
 

@@ -1380,7 +1380,15 @@ int j1j2_run(void *handle, int32_t phase, int32_t ncycles)
     if (!E) return fail(J1J2_EINVAL, "null engine handle");
     if (ncycles < 0) ncycles = phase == J1J2_PHASE_MEAS ? E->common.nmess : E->common.neq;
     HIPCHK(hipSetDevice(E->device));
-    return run_cycles(*E, phase, ncycles, 1);
+    int rc = run_cycles(*E, phase, ncycles, 1);
+    if (rc == 0 && phase == J1J2_PHASE_MEAS && ncycles == 0 && E->locnem) {
+        // a measuring phase of no cycles still takes one local-nematic sample
+        // of the current state (NMESS <= 0 branches, J1J2_6tv.f90:1414-1424
+        // for PT, 2157-2160 for LOCAL)
+        HIPCHK(launch_locnem(E->st, E->max_n, 0, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+    }
+    return rc;
 }
 
 int j1j2_get_spins(void *handle, int32_t conf, int8_t *spins, int32_t nrep)

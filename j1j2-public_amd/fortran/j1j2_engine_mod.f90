@@ -235,6 +235,14 @@ module j1j2_engine_mod
          integer(c_int32_t), value :: len
       end function
 
+      ! no counterpart in the reference: for callers that cannot allocate
+      integer(c_int) function j1j2_run_cycles_noalloc(handle, phase, ncycles, begin, stream) &
+         bind(C, name='j1j2_run_cycles_noalloc')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: handle, stream
+         integer(c_int32_t), value :: phase, ncycles, begin
+      end function
+
       integer(c_int) function j1j2_synchronize(handle) bind(C, name='j1j2_synchronize')
          import :: c_int, c_ptr
          type(c_ptr), value :: handle

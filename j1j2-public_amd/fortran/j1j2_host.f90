@@ -335,17 +335,27 @@ contains
    ! One PT phase of `total` cycles in chunks of max(1, total/10), with the
    ! reference's progress reports (J1J2_6tv.f90:1267-1276, 1430-1531); the
    ! chunks continue one run (the first begins the phase), so the result is
-   ! that of a single j1j2_run.
+   ! that of a single j1j2_run.  The first chunk holds at least TUNE_CYCLES
+   ! cycles (a whole number of tenths) when the phase has that many, so that
+   ! the engine's autotuner, which times its candidates on the first call long
+   ! enough for them, gets to run.  A phase of no cycles is one j1j2_run, which
+   ! samples the state as the reference's NMESS <= 0 branch does.
    subroutine run_pt_phase(eng, phase, label, iconf, total)
       type(c_ptr), intent(in) :: eng
       integer(c_int), intent(in) :: phase
       character(len=*), intent(in) :: label
       integer, intent(in) :: iconf, total
+      integer, parameter :: TUNE_CYCLES = 40
       integer :: every, done, chunk
       every = max(1, total/10)
       call write_progress(label, iconf, 0, total)
+      if (total <= 0) then
+         call j1j2_check(j1j2_run(eng, phase, 0_i4b), label)
+         return
+      end if
       done = 0
       chunk = min(every, total)
+      if (total >= TUNE_CYCLES .and. chunk < TUNE_CYCLES) chunk = min(total, every*((TUNE_CYCLES + every - 1)/every))
       call j1j2_check(j1j2_run_cycles(eng, phase, int(chunk, i4b), 1_i4b), label)
       done = chunk
       if (done > 0) call write_progress(label, iconf, done, total)

@@ -533,6 +533,28 @@ void ora_pt_free(ora_pt *pt)
 
 void ora_pt_begin_phase(ora_pt *pt) { pt->swap_phase = 1; }
 
+void ora_pt_reset_measurement(ora_pt *pt)
+{
+    /* the sums, extremes, histogram and visited flags zeroed at the start of
+     * Pt_measurement_loop (J.f90:1375-1399, 1402-1403), Reset_local_nematic_
+     * average (J.f90:1000-1006), and the LOCAL series of Measurement_loop */
+    int32_t nrep = pt->nrep, N = pt->lat.N;
+    memset(pt->sums, 0, sizeof(double) * (size_t)nrep * ORA_NSUMS);
+    for (int32_t i = 0; i < nrep; ++i) pt->best[i] = 1.7976931348623157e308;
+    memset(pt->final_en, 0, sizeof(double) * (size_t)nrep);
+    memset(pt->hist, 0, sizeof(int64_t) * (size_t)nrep * pt->bins);
+    memset(pt->hmin, 0, sizeof(double) * (size_t)nrep);
+    memset(pt->hmax, 0, sizeof(double) * (size_t)nrep);
+    memset(pt->hbin, 0, sizeof(double) * (size_t)nrep);
+    memset(pt->vcold, 0, (size_t)nrep);
+    memset(pt->vhot, 0, (size_t)nrep);
+    memset(pt->locspin, 0, sizeof(int64_t) * (size_t)N);
+    memset(pt->locnem, 0, sizeof(int64_t) * (size_t)N);
+    pt->nmeasure = 0;
+    pt->locsamples = 0;
+    pt->series_len = 0;
+}
+
 static void swap_pass(ora_pt *pt)
 {
     /* Pt_attempt_swaps, J.f90:1303-1338, with the engine's draw */

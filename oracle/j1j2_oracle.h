@@ -151,6 +151,7 @@ int  ora_pt_init(ora_pt *pt, int32_t L, int32_t H, double J1, double deltaJ1, do
                  int32_t count_flips, int32_t shadow);
 void ora_pt_free(ora_pt *pt);
 void ora_pt_begin_phase(ora_pt *pt);   /* swap_phase = 1 (J.f90:1266, 1425) */
+void ora_pt_reset_measurement(ora_pt *pt);   /* start of a measuring phase */
 void ora_pt_cycle(ora_pt *pt, int measure);
 /* Derived per-slot fields, 49 per slot, layout of J.f90:2937-2985 (one config),
  * out[f*nrep + slot]. */

@@ -90,6 +90,7 @@ def _declare(L):
                               _i32, _i32, _i32, _i32, _u32, _u32, _P, _i64, _i32, _i32]
     L.ora_pt_free.argtypes = [ctypes.POINTER(PT)]
     L.ora_pt_begin_phase.argtypes = [ctypes.POINTER(PT)]
+    L.ora_pt_reset_measurement.argtypes = [ctypes.POINTER(PT)]
     L.ora_pt_cycle.argtypes = [ctypes.POINTER(PT), ctypes.c_int]
     L.ora_pt_basefields.argtypes = [ctypes.POINTER(PT), _P]
     L.ora_pt_hist.argtypes = [ctypes.POINTER(PT), _P]
@@ -272,6 +273,21 @@ class OraclePT:
 
     def begin_phase(self):
         lib().ora_pt_begin_phase(ctypes.byref(self.s))
+
+    def reset_measurement(self):
+        lib().ora_pt_reset_measurement(ctypes.byref(self.s))
+
+    def set_spins(self, spins):
+        """Overwrite every replica's spins (a fresh start; the sweep clock runs on)."""
+        s = np.ascontiguousarray(spins, np.int8).reshape(-1)
+        assert s.size == self.nrep * self.N
+        ctypes.memmove(self.s.spins, s.ctypes.data, s.size)
+
+    def set_temperatures(self, temps):
+        """New temperature per slot; the chain carries on from its current state."""
+        b = 1.0 / np.ascontiguousarray(temps, np.float64)
+        assert b.size == self.nrep
+        ctypes.memmove(self.s.beta, b.ctypes.data, b.nbytes)
 
     def cycle(self, measure):
         lib().ora_pt_cycle(ctypes.byref(self.s), 1 if measure else 0)

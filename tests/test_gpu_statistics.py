@@ -26,7 +26,7 @@ import pytest
 import j1j2_engine as J
 import oracle as O
 from gpu_cases import config_inputs, say
-from refdata import find, read_table
+from refdata import find, read_cfhs, read_cfpt, read_table
 
 pytestmark = pytest.mark.gpu
 
@@ -96,56 +96,251 @@ def test_pt_matches_reference_l8(golden_dir):
             f"z {np.array2string(z, precision=2)}")
     frac = zsummary(zs, "pt_l8_vs_reference")
     assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
+    # every replica of the reference's run visited both ends of the ladder; so
+    # does every replica of every engine chain of the same length
+    visit = read_cfpt(find(os.path.join(golden_dir, "ref_pt_l8_stat"), "cfpt"))[3]
+    assert [o["visitboth"] for o in outs] == [visit] * NCHAIN
+    hz = histogram_vs_reference(outs, os.path.join(golden_dir, "ref_pt_l8_stat"))
+    frac = zsummary(hz, "pt_l8_histogram_vs_reference")
+    assert np.all(np.abs(hz) < 4.0) and frac >= 0.75
 
 
-def test_local_matches_reference_l8(golden_dir):
-    """Reference run ref_local_l8 (the reference's CPU-runnable case): LOCAL, L = 8, T = 10, HOT
-    start, NEQ = NMESS = 1000 sweeps.  The start is the reference's: KISS seeded
-    with IRINIT = 1, 3*L^3 draws consumed by Setup_vacancies (J1J2_6tv.f90:
-    1116-1175), then Initialize_spins."""
-    say("case LOCAL L=8 vs reference (64 chains)")
+def histogram_vs_reference(outs, gd):
+    """The engine chains' energy histograms against the reference's (cfhs).
+    Each histogram has its own range (first measured energy +- (|e0| + 1)),
+    so they are compared as distributions: the cumulative distribution at
+    the reference's bin edges nearest its deciles, the engine's read off a
+    monotone cubic through its own bin edges (interpolation error ~2e-4),
+    z = (reference - chain mean) / (chain spread * sqrt(1 + 1/64)).  Also:
+    the labels (lower bin edges) average to <e> - width/2, the rule of
+    1 + int((e - min) / width) binning that the reference's file obeys (NINT
+    binning would put them at <e> - width): (label mean + width/2 - <e>) /
+    width within 0.2 for every chain and 0.05 over the chains."""
+    from scipy.interpolate import PchipInterpolator
+    ref = read_cfhs(find(gd, "cfhs"))
+    zs = []
+    for s, (T, bins, labels, freq) in enumerate(ref):
+        w_r = (labels[-1] - labels[0]) / (len(labels) - 1)
+        edges_r = labels[0] + w_r * np.arange(len(labels) + 1)
+        F_r = np.concatenate([[0.0], np.cumsum(freq)])
+        ks = sorted({int(np.argmin(np.abs(F_r - q))) for q in np.arange(1, 10) / 10.0})
+        x = edges_r[ks]
+        F, off = [], []
+        for o in outs:
+            h, lo, w = o["hist"][:, s], o["hist_min"][s], o["hist_bin"][s]
+            edges = lo + w * np.arange(h.size + 1)
+            F.append(PchipInterpolator(edges, np.concatenate([[0.0], np.cumsum(h)]), extrapolate=True)(x))
+            off.append((float((h * edges[:-1]).sum()) + w / 2 - o["basefields"][9][s]) / w)
+        off = np.array(off)
+        assert np.all(np.abs(off) < 0.2) and abs(off.mean()) < 0.05, (s, off.mean(), off.min(), off.max())
+        F = np.clip(np.array(F), 0.0, 1.0)
+        mu, sd = F.mean(0), F.std(0, ddof=1)
+        z = (F_r[ks] - mu) / np.sqrt(sd ** 2 * (1 + 1.0 / len(outs)) + 1e-300)
+        say(f"  histogram T={T:.3f}: CDF ref {np.array2string(F_r[ks], precision=3)} z {np.array2string(z, precision=2)}")
+        zs.extend(z.tolist())
+    return zs
+
+
+@pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
+def test_frozen_pt_run_matches_reference(golden_dir, precision):
+    """Reference run ref_pt_frozen: L = 8, NREPLICA = 5, T = 0.01 .. 0.02, FER
+    start, phi = 0, NEQ = 11, NMESS = 7.  Nothing flips, every energy is -1 and
+    every swap is accepted, so the run is deterministic: the engine must
+    reproduce the histogram (all in bin 51 of [-3, 1]), the swap attempts and
+    accepts per pair, the visited-both-ends count and the energies exactly."""
+    say(f"case frozen PT run vs reference, precision {precision}")
+    gd = os.path.join(golden_dir, "ref_pt_frozen")
+    L, R = 8, 5
+    g = J.make_params(L, L, nreplica=R, tmin=0.01, tmax=0.02, neq=11, nmess=7, precision=precision, seed=4)
+    eng = J.Engine([g])
+    eng.upload_disorder(0, np.zeros(L ** 3))
+    s0 = np.tile(J.initial_spins(L, L, J.START_FER, 0, 0), R)
+    eng.set_spins(0, s0)
+    eng.run(J.PHASE_EQUIL)
+    eng.run(J.PHASE_MEAS)
+    out = eng.fetch(0)
+    assert np.array_equal(eng.spins(0).reshape(-1), s0)
+    eng.close()
+    att, acc, rate, visit, frac = read_cfpt(find(gd, "cfpt"))
+    assert np.array_equal(out["swap_att"], att) and np.array_equal(out["swap_acc"], acc)
+    assert out["visitboth"] == visit
+    for s, (T, bins, labels, freq) in enumerate(read_cfhs(find(gd, "cfhs"))):
+        w = (labels[-1] - labels[0]) / (len(labels) - 1)
+        assert abs(out["hist_min"][s] - labels[0]) < 1e-6 and abs(out["hist_bin"][s] - w) < 1e-7
+        assert np.array_equal(out["hist"][:99, s], freq)
+    assert np.array_equal(out["basefields"][9], read_table(find(gd, "cfen"))[:, 1])
+
+
+@pytest.mark.parametrize("precision", [J.FP64_PLAIN, J.FP32_BATCHED])
+def test_local_matches_reference_l8(golden_dir, precision):
+    """Reference run ref_local_l8 (the reference's CPU-runnable case, an FP64
+    run): LOCAL, L = 8, T = 10, HOT start, NEQ = NMESS = 1000 sweeps, in both
+    forms.  The start is the reference's: KISS seeded with IRINIT = 1, 3*L^3
+    draws consumed by Setup_vacancies (J1J2_6tv.f90:1116-1175), then
+    Initialize_spins."""
+    say(f"case LOCAL L=8 vs reference (64 chains), precision {precision}")
     L = 8
     N = L * L * L
     spins = J.initial_spins(L, L, J.START_HOT, 0, 1, skip=3 * N)
     assert np.array_equal(spins, np.where(O.kiss_stream(1, 4 * N)[3 * N:] < 0.5, 1, -1))
     g = J.make_params(L, L, nreplica=1, ptswapever=1, sampler=J.SAMPLER_LOCAL, ladder=J.LADDER_LOCAL,
-                      tmin=10.0, tmax=10.0, dt=1.0, neq=1000, nmess=1000, seed=4242)
+                      tmin=10.0, tmax=10.0, dt=1.0, neq=1000, nmess=1000, seed=4242, precision=precision)
     p = J.make_disorder(L, L, 3.0, 1, True, 2.0)
     zs, rows, _ = chains_vs_reference(g, p, spins, "local_l8", golden_dir)
     for name, ref, mu_, z in rows:
         say(f"  {name:7s} ref {np.array2string(ref, precision=5)} engine {np.array2string(mu_, precision=5)} "
             f"z {np.array2string(z, precision=2)}")
-    frac = zsummary(zs, "local_l8_vs_reference")
+    frac = zsummary(zs, f"local_l8_vs_reference_{'fp64' if precision == J.FP64_PLAIN else 'fp32'}")
     assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
 
 
-def test_fp32_matches_fp64_l32():
-    """FP32 batched form vs FP64 plain form at L = 32, 16 replicas x 64 configs."""
-    say("case FP32 vs FP64 at L=32")
-    L, R, C = 32, 16, 64
+def paired_z(d):
+    """Paired differences d (pairs along axis 0): the t statistic of their mean,
+    expressed as the normal z-score of the same two-sided p-value (t with
+    n - 1 degrees of freedom), so that few pairs (8 configs at L = 96) are
+    judged as fairly as many."""
+    from scipy.stats import norm, t as tdist
+    n = d.shape[0]
+    sd = d.std(0, ddof=1)
+    tt = np.where(sd > 0, d.mean(0) / (sd / np.sqrt(n) + 1e-300), 0.0)
+    p = 2.0 * tdist.sf(np.abs(tt), n - 1)
+    return np.sign(tt) * norm.isf(np.maximum(p, 1e-300) / 2.0)
+
+
+def fp32_vs_fp64(label, groups, neq, nmess, same_disorder=None, sampler=J.SAMPLER_PT, temps=(8.0, 11.5)):
+    """Both forms on the same configs, Philox keys and starts: groups are
+    (L, nreplica, nconf, start); the FP32 run also counts, per config, the
+    proposals whose FP32 decision differs from the FP64 rule's.  Returns the
+    paired z-scores (per group, quantity and slot, and per swap pair) and
+    the per-config mismatch counts."""
     res = {}
     for prec in (J.FP32_BATCHED, J.FP64_PLAIN):
-        g = J.make_params(L, L, nreplica=R, nconf=C, precision=prec, neq=300, nmess=1500, seed=99)
-        eng = J.Engine([g])
-        for c in range(C):
-            eng.upload_disorder(c, J.make_disorder(L, L, 3.0, 1 + c, True, 2.0))
-            eng.set_spins(c, np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R))
-        eng.run(J.PHASE_EQUIL, g.neq)
-        eng.run(J.PHASE_MEAS, g.nmess)
-        outs = [eng.fetch(c) for c in range(C)]
-        res[prec] = (np.stack([o["basefields"] for o in outs]),
-                     np.stack([o["swap_acc"] / o["swap_att"] for o in outs]))
+        params, iconf = [], 1
+        for L, R, C, _ in groups:
+            params.append(J.make_params(L, L, nreplica=R, nconf=C, iconf0=iconf, precision=prec, neq=neq, nmess=nmess,
+                                        seed=99, sampler=sampler, tmin=temps[0], tmax=temps[1],
+                                        ptswapever=5 if sampler == J.SAMPLER_PT else 1,
+                                        ladder=J.LADDER_LINEAR if sampler == J.SAMPLER_PT else J.LADDER_LOCAL,
+                                        flags=J.FLAG_SHADOW_FP64 if prec == J.FP32_BATCHED else 0))
+            iconf += C
+        eng = J.Engine(params)
+        c = 0
+        for (L, R, C, start), g in zip(groups, params):
+            for k in range(C):
+                if same_disorder is not None:
+                    p, sp = same_disorder
+                else:
+                    p, sp = config_inputs(L, L, R, g.iconf0 + k, "hot" if start == "mixed" and k % 2 == 0 else
+                                          ("str" if start == "mixed" else start))
+                eng.upload_disorder(c, p)
+                eng.set_spins(c, np.asarray(sp).reshape(-1))
+                c += 1
+        eng.run(J.PHASE_EQUIL, neq)
+        eng.run(J.PHASE_MEAS, nmess)
+        outs = [eng.fetch(c) for c in range(eng.nconf)]
+        mis = [eng.mismatch(c) for c in range(eng.nconf)] if prec == J.FP32_BATCHED else None
+        res[prec] = (outs, mis)
         eng.close()
-    zs = []
-    for name, f, _, _ in QUANTITIES:
-        d = res[J.FP32_BATCHED][0][:, f, :] - res[J.FP64_PLAIN][0][:, f, :]
-        z = d.mean(0) / (d.std(0, ddof=1) / np.sqrt(C) + 1e-300)
-        zs.extend(z.tolist())
-        say(f"  {name:7s} max |z| over slots {np.max(np.abs(z)):.2f}")
-    d = res[J.FP32_BATCHED][1] - res[J.FP64_PLAIN][1]
-    zs.extend((d.mean(0) / (d.std(0, ddof=1) / np.sqrt(C))).tolist())
-    frac = zsummary(zs, "fp32_vs_fp64_l32")
+    zs, c0 = [], 0
+    for L, R, C, _ in groups:
+        sl = slice(c0, c0 + C)
+        b32 = np.stack([o["basefields"] for o in res[J.FP32_BATCHED][0][sl]])
+        b64 = np.stack([o["basefields"] for o in res[J.FP64_PLAIN][0][sl]])
+        for name, f, _, _ in QUANTITIES:
+            d = b32[:, f, :] - b64[:, f, :]
+            zs.extend(paired_z(d).tolist())
+        if sampler == J.SAMPLER_PT and R > 1:
+            a32 = np.stack([o["swap_acc"] / o["swap_att"] for o in res[J.FP32_BATCHED][0][sl]])
+            a64 = np.stack([o["swap_acc"] / o["swap_att"] for o in res[J.FP64_PLAIN][0][sl]])
+            zs.extend(paired_z(a32 - a64).tolist())
+        c0 += C
+    mis = res[J.FP32_BATCHED][1]
+    prop = np.array([m[0] for m in mis], np.float64)
+    bad = np.array([m[1] for m in mis], np.float64)
+    say(f"  {label}: flip-decision mismatch {bad.sum():.0f} of {prop.sum():.3g} proposals "
+        f"({bad.sum() / prop.sum():.3g}); per config {np.min(bad / prop):.3g} .. {np.max(bad / prop):.3g}")
+    if OUT:
+        os.makedirs(OUT, exist_ok=True)
+        with open(os.path.join(OUT, f"mismatch_{label}.json"), "w") as f:
+            json.dump({"proposals": prop.tolist(), "mismatches": bad.tolist()}, f)
+    assert np.all(prop > 0) and bad.sum() / prop.sum() < 1e-6
+    return zs
+
+
+def judge(zs, label):
+    frac = zsummary(zs, label)
     assert np.all(np.abs(zs) < 4.0) and frac >= 0.85
+
+
+def test_fp32_matches_fp64_c1_local_l8():
+    """c1 shape: LOCAL, L = 8, T = 10, the reference's HOT start and disorder,
+    64 chains (independent Philox keys), 1000 + 1000 sweeps."""
+    say("case FP32 vs FP64, c1 shape (LOCAL L=8)")
+    L, N = 8, 512
+    same = (J.make_disorder(L, L, 3.0, 1, True, 2.0), J.initial_spins(L, L, J.START_HOT, 0, 1, skip=3 * N))
+    judge(fp32_vs_fp64("c1_local_l8", [(8, 1, 64, None)], 1000, 1000, same_disorder=same,
+                       sampler=J.SAMPLER_LOCAL, temps=(10.0, 10.0)), "fp32_vs_fp64_c1")
+
+
+def test_fp32_matches_fp64_c2_l16():
+    """c2 shape: PT, L = 16, 16 replicas, STR-X start, the disorder of IRINIT = 1,
+    64 chains (independent Philox keys), NEQ = NMESS = 2000."""
+    say("case FP32 vs FP64, c2 shape (L=16 16 replicas)")
+    L, R = 16, 16
+    same = (J.make_disorder(L, L, 3.0, 1, True, 2.0), np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R))
+    judge(fp32_vs_fp64("c2_pt_l16", [(16, 16, 64, None)], 2000, 2000, same_disorder=same), "fp32_vs_fp64_c2")
+
+
+def test_fp32_matches_fp64_l32():
+    """c3: L = 32, 16 replicas x 64 disorder configs, STR-X start."""
+    say("case FP32 vs FP64 at L=32")
+    judge(fp32_vs_fp64("c3_l32", [(32, 16, 64, "str")], 300, 1500), "fp32_vs_fp64_l32")
+
+
+def test_fp32_matches_fp64_c4_uneven():
+    """c4: L in {16, 24, 32, 48} x 16 configs, 8/12/16/24 replicas, HOT and STR
+    starts alternating; paired per size over its 16 configs."""
+    say("case FP32 vs FP64, c4 uneven batch")
+    judge(fp32_vs_fp64("c4_uneven", [(16, 8, 16, "mixed"), (24, 12, 16, "mixed"), (32, 16, 16, "mixed"),
+                                     (48, 24, 16, "mixed")], 300, 1500), "fp32_vs_fp64_c4")
+
+
+def test_fp32_matches_fp64_c5_l96():
+    """c5: L = 96, 24 replicas x 8 configs, STR-X start; paired over the 8 configs."""
+    say("case FP32 vs FP64, c5 L=96")
+    judge(fp32_vs_fp64("c5_l96", [(96, 24, 8, "str")], 200, 600), "fp32_vs_fp64_c5")
+
+
+def test_pt_matches_reference_c2(golden_dir):
+    """Reference run ref_pt_l16_c2 (the c2 configuration, run by the reference
+    on the CPU in FP64): L = 16, 16 replicas over T = 8.0-11.5, STR-X start,
+    NEQ = NMESS = 2000 cycles, PTSWAPEVER = 5, disorder IRINIT = 1.  64 chains of
+    the plain FP64 form on the same disorder: every quantity per slot, the swap
+    acceptances, the visited-both-ends count and the energy histograms."""
+    say("case PT L=16 (c2) FP64 plain vs reference (64 chains)")
+    L, R = 16, 16
+    gd = os.path.join(golden_dir, "ref_pt_l16_c2")
+    g = J.make_params(L, L, nreplica=R, neq=2000, nmess=2000, seed=1618, precision=J.FP64_PLAIN)
+    p = J.make_disorder(L, L, 3.0, 1, True, 2.0)
+    spins = np.tile(J.initial_spins(L, L, J.START_STR, 0, 0), R)
+    zs, rows, outs = chains_vs_reference(g, p, spins, "pt_l16_c2", golden_dir)
+    att, acc, rate, visit, frac = read_cfpt(find(gd, "cfpt"))
+    assert all(np.array_equal(o["swap_att"], att) for o in outs)
+    a = np.stack([o["swap_acc"] / o["swap_att"] for o in outs])
+    zs.extend(((rate - a.mean(0)) / np.sqrt(a.std(0, ddof=1) ** 2 * (1 + 1.0 / NCHAIN))).tolist())
+    v = np.array([o["visitboth"] for o in outs], np.float64)
+    if v.std() > 0:
+        zs.append(float((visit - v.mean()) / (v.std(ddof=1) * np.sqrt(1 + 1.0 / NCHAIN))))
+    else:
+        assert visit == v[0]
+    say(f"  visited both ends: reference {visit}, engine {v.mean():.2f} +- {v.std():.2f}")
+    for name, ref, mu_, z in rows:
+        say(f"  {name:7s} max |z| {np.max(np.abs(z)):.2f}")
+    frac = zsummary(zs, "pt_l16_c2_vs_reference")
+    assert np.all(np.abs(zs) < 4.0) and frac >= 0.75
+    hz = histogram_vs_reference(outs, gd)
+    frac = zsummary(hz, "pt_l16_c2_histogram_vs_reference")
+    assert np.all(np.abs(hz) < 4.0) and frac >= 0.75
 
 
 @pytest.mark.parametrize("precision", [J.FP32_BATCHED, J.FP64_PLAIN])
