@@ -299,6 +299,30 @@ def other_workload(J, name, steps, warmup, with_fp64=True, with_cpu=True):
     }
 
 
+def copy_probe(nbytes, reps=5):
+    """Five plain device-to-device copies of nbytes (the runtime's copy
+    kernel): the known traffic that calibrates the FETCH_SIZE / WRITE_SIZE
+    counters."""
+    import ctypes
+    J.lib()                                        # the runtime the engine uses
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    for buf in (src, dst):
+        if hip.hipMalloc(ctypes.byref(buf), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, n) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(reps):
+            if hip.hipMemcpy(dst, src, n, 3) != 0:     # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,24 +337,23 @@ def main():
                     help="also run 5 plain device copies of this many bytes (PMC counter calibration)")
     ap.add_argument("--block-threads", type=int, default=0,
                     help="fix the resident sweep's threads per block (no autotuning), e.g. for counter runs")
+    ap.add_argument("--bit-slabs", type=int, default=0,
+                    help="fix the z-slabs per bit-layout lattice (no autotuning), e.g. for counter runs")
     args = ap.parse_args()
     if args.block_threads:
         os.environ["J1J2_RES_THREADS"] = str(args.block_threads)
+    if args.bit_slabs:
+        os.environ["J1J2_BIT_SLABS"] = str(args.bit_slabs)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: measures one GPU only; the path would shard disorder configs across GPUs "
               "(one process per GPU, each its own engine over its share of the configs, no exchange "
               "during the run, results gathered once at the end).", file=sys.stderr)
         sys.exit(2)
+    global J
     import j1j2_engine as J
 
     if args.copy_probe:
-        import torch
-        src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_probe(args.copy_probe)
 
     if args.workload != "c3":
         out = other_workload(J, args.workload, args.steps, args.warmup, not args.no_fp64, not args.no_cpu)
