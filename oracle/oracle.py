@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libj1j2_oracle.so")
+# J1J2_ORACLE_LIB names another build of the same sources (the sanitizer test's)
+LIB_PATH = os.environ.get("J1J2_ORACLE_LIB") or os.path.join(_HERE, "_build", "libj1j2_oracle.so")
 
 _lib = None
 
