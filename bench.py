@@ -60,12 +60,12 @@ def warm(J, eng, warmup):
     eng.synchronize()
 
 
-def time_form(J, precision, steps, warmup, kernel_timing, nconf=NCONF):
+def time_form(J, precision, steps, warmup, kernel_timing, nconf=NCONF, flags=0):
     """Timed region: `steps` MEAS cycles replayed as hipGraphs (both graphs
     built during the warm-up).  With kernel_timing, a second pass of the same
     length brackets every sweep launch with HIP events on the engine's stream
     (launched eagerly) for the sweep kernel's average duration."""
-    eng = build_batch(J, precision, nconf=nconf)
+    eng = build_batch(J, precision, flags=flags, nconf=nconf)
     warm(J, eng, warmup)
     t0 = time.perf_counter()
     eng.run_cycles(J.PHASE_MEAS, steps, begin=False)
@@ -337,6 +337,8 @@ def main():
                     help="also run 5 plain device copies of this many bytes (PMC counter calibration)")
     ap.add_argument("--block-threads", type=int, default=0,
                     help="fix the resident sweep's threads per block (no autotuning), e.g. for counter runs")
+    ap.add_argument("--flips", action="store_true",
+                    help="also time the L=32 batch with per-site flip counts (WRITENUMFLIPS)")
     ap.add_argument("--bit-slabs", type=int, default=0,
                     help="fix the z-slabs per bit-layout lattice (no autotuning), e.g. for counter runs")
     args = ap.parse_args()
@@ -418,6 +420,10 @@ def main():
                                                                 "loops over them (an engine of 16 replicas)"}
         rate, n = mismatch_rate(J, 4)
         res["flip_mismatch_rate"] = {"fp32_vs_fp64": rate, "proposals": n}
+    if args.flips:
+        fl = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=False, flags=J.FLAG_WRITENUMFLIPS)
+        res["with_flip_counts"] = {"value": fl["rate"], "sweep_path": J.PATH_NAMES[fl["path"]],
+                                   "over_value": fl["rate"] / fp32["rate"]}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()
     print(json.dumps(res), flush=True)
