@@ -1034,9 +1034,13 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         G.conf0 = (int32_t)E->confs.size();
         G.lat0 = (int32_t)nlat;
         G.nlat = p.nconf * p.nreplica;
-        const bool may = p.precision == J1J2_FP32_BATCHED && !flips && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
-        if (may && resident_supported(p.L) && (size_t)G.N <= lds_limit) G.kind = 1;
-        else if (may && bits_supported(p.L) && bits_lds_need(p.L, p.H) <= lds_limit + RESIDENT_SCRATCH_LDS)
+        // flip counts (WRITENUMFLIPS) keep a byte per site in LDS beside the
+        // byte layout (counts per launch <= PTSWAPEVER); the bit layout has no
+        // room for them, so such lattices take the global-memory sweep
+        const bool may = p.precision == J1J2_FP32_BATCHED && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
+        const size_t res_need = (size_t)G.N * (flips ? 2 : 1);
+        if (may && resident_supported(p.L) && res_need <= lds_limit && (!flips || p.ptswapever <= 255)) G.kind = 1;
+        else if (may && !flips && bits_supported(p.L) && bits_lds_need(p.L, p.H) <= lds_limit + RESIDENT_SCRATCH_LDS)
             G.kind = 2;
         G.resident = G.kind != 0;
         E->all_resident = E->all_resident && G.resident;
@@ -1102,8 +1106,10 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         E->autotune = false;
     }
     if (const char *at = std::getenv("J1J2_AUTOTUNE")) E->autotune = E->autotune && std::atoi(at) != 0;
-    if (std::any_of(E->groups.begin(), E->groups.end(), [](const Group &g) { return g.kind == 1; }))
-        HIPCHK(setup_resident((size_t)E->max_n));
+    size_t byte_lds = 0;                           // the byte-layout launches' largest lattice (and counts)
+    for (const Group &G : E->groups)
+        if (G.kind == 1) byte_lds = std::max(byte_lds, (size_t)G.N * (flips ? 2 : 1));
+    if (byte_lds) HIPCHK(setup_resident(byte_lds));
     size_t bits_lds = 0;
     for (const Group &G : E->groups)
         if (G.kind == 2) bits_lds = std::max(bits_lds, bits_lds_need(G.p.L, G.p.H));

@@ -79,7 +79,10 @@ struct DeviceState {
 // every lattice and resident_nt_valid(L, nt).  L: the common lattice size of
 // the segment, or 0 for mixed sizes; H: the common height when L > 0 (0 if
 // the heights differ), and for mixed sizes -1 if every lattice has H = L (0
-// otherwise).  The result depends neither on nt nor on the order.
+// otherwise).  The result depends neither on nt nor on the order.  With flip
+// counts (st.flips set) each block keeps a byte per site after its lattice
+// in LDS (2 max_n bytes; PTSWAPEVER <= 255) and adds them to the slot's
+// counters at the end of the launch.
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord,
                                  int32_t max_n, int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);

@@ -418,6 +418,8 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
             if (((open >> j) & 1u) && decide_f32_open(ys[j], ra[j])) flip |= 4u << (8 * j);
     }
     *P.own = S ^ flip;
+    // flip counts (WRITENUMFLIPS): this thread alone touches these four sites' counters in this pass
+    if (cx.cnt_off) P.own[cx.cnt_off] += flip >> 2;
 }
 
 // One colour pass over plane c.  HC > 0: the lattice height is HC, known at
@@ -540,6 +542,11 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
             reinterpret_cast<uint4 *>(lds32)[2 * i + 1] = make_uint4(b[4], b[5], b[6], b[7]);
         }
     }
+    // flip counts of this launch, one byte per site right after the lattice
+    // (at most PTSWAPEVER <= 255 per site), added to the slot's counters at the end
+    const bool counting = st.flips != nullptr;
+    if (counting)
+        for (int i = tid; i < N / 4; i += NT) lds32[N / 4 + i] = 0u;
 
     const ConfDesc cd = st.confs[d.conf];
     const ConfConst cc = st.cconst[d.conf];
@@ -557,6 +564,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
+    cx.cnt_off = counting ? N / 4 : 0;
     const uint64_t S0 = clock_cycle(st) * (uint64_t)st.ptswapever;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
@@ -578,6 +586,20 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
             if (c & 1) colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, c, H, sh_prop, sh_mis);
             else colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, c, H, sh_prop, sh_mis);
             __syncthreads();
+        }
+    }
+    if (counting) {
+        // confnumflips(slot, site) += this launch's counts (internal site order)
+        int4 *fl = reinterpret_cast<int4 *>(st.flips + cd.flip_off + (int64_t)slot * N);
+        for (int i = tid; i < N / 4; i += NT) {
+            const uint32_t n4 = lds32[N / 4 + i];
+            if (!n4) continue;
+            int4 g = fl[i];
+            g.x += (int)(n4 & 0xFFu);
+            g.y += (int)((n4 >> 8) & 0xFFu);
+            g.z += (int)((n4 >> 16) & 0xFFu);
+            g.w += (int)(n4 >> 24);
+            fl[i] = g;
         }
     }
 
@@ -653,7 +675,8 @@ void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
     const int lat = st.order[ord0 + blockIdx.x];
     if (lat < 0) return;
     const LatDesc d = st.lats[lat];
-    resident_body<WPH, NT, SHADOW, HC>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4));
+    const int lds_words = (st.flips ? 2 : 1) * (d.N / 4);     // lattice (and flip counts)
+    resident_body<WPH, NT, SHADOW, HC>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + lds_words));
 }
 
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
@@ -666,7 +689,7 @@ void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
     const int lat = st.order[ord0 + blockIdx.x];
     if (lat < 0) return;
     const LatDesc d = st.lats[lat];
-    int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + d.N / 4);
+    int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + (st.flips ? 2 : 1) * (d.N / 4));
     switch (d.L) {
         case 8: resident_body<1, NT, SHADOW, CUBIC ? 8 : 0>(st, d, lat, lds32, red); break;
         case 16: resident_body<2, NT, SHADOW, CUBIC ? 16 : 0>(st, d, lat, lds32, red); break;
@@ -1121,7 +1144,8 @@ hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t no
     if (nord <= 0) return hipSuccess;
     const ResKernel k = pick_resident(L, nt, shadow, L > 0 ? H == L : H < 0);
     if (!k) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k, dim3(nord), dim3(nt), (size_t)max_n + RES_STATIC_LDS, s, st, ord0, nord);
+    const size_t lattice = (size_t)max_n * (st.flips ? 2 : 1);      // with flip counts: a byte per site more
+    hipLaunchKernelGGL(k, dim3(nord), dim3(nt), lattice + RES_STATIC_LDS, s, st, ord0, nord);
     return hipGetLastError();
 }
 

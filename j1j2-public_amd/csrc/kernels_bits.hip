@@ -483,6 +483,7 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
+    cx.cnt_off = 0;                                            // flip counts take the global-memory sweep
     const uint64_t S0 = clock_cycle(st) * (uint64_t)st.ptswapever;
     // exchange versions count the lattice's slab launches (word 0 of slab 0's
     // observable slots, advanced by slab 0 once every slab has published its

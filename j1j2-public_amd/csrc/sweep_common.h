@@ -87,6 +87,8 @@ struct SweepCtx {
     float b2, j2d, j3d;        // j2d = 2^24 (2 J2 / 4), j3d = 2^24 (2 J3 / 4) (exact scalings)
     double beta;
     uint32_t ctr1, sweep_lo, sweep_hi, key0, key1;
+    int cnt_off;               // resident sweep with flip counts: dwords from a lattice word to its
+                               // counter word (one byte per site); 0: no counts
 };
 
 

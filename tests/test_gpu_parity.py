@@ -211,10 +211,27 @@ def test_shadow_mismatch_counts_match_oracle():
         run_and_compare([g], "hot", 2, 2, shadow=True)
 
 
-def test_flip_counts_match_oracle():
-    say("case WRITENUMFLIPS")
-    g = J.make_params(8, 4, nreplica=3, nconf=1, flags=J.FLAG_WRITENUMFLIPS, seed=SEED)
-    run_and_compare([g], "hot", 2, 3, flips=True)
+@pytest.mark.parametrize("flags,path", [(0, 1), (J.FLAG_FORCE_GLOBAL, 2), (J.FLAG_SHADOW_FP64, 1)])
+def test_flip_counts_match_oracle(flags, path):
+    """WRITENUMFLIPS: per-slot, per-site flip counts (confnumflips) on the
+    LDS-resident sweep (a byte per site beside the lattice, added to the
+    slot's counters at the end of each launch), on the global-memory sweep,
+    and with the FP64 shadow."""
+    say(f"case WRITENUMFLIPS flags {flags}")
+    g = J.make_params(8, 4, nreplica=3, nconf=2, flags=J.FLAG_WRITENUMFLIPS | flags, seed=SEED)
+    eng, _ = run_and_compare([g], "hot", 2, 3, flips=True, shadow=bool(flags & J.FLAG_SHADOW_FP64))
+    assert eng.sweep_path()[0] == path
+
+
+def test_flip_counts_uneven_batch_match_oracle():
+    """Flip counts over an uneven batch: L = 16, 24 and 40 on the LDS-resident
+    sweep (lattice and counts fit one CU's LDS), L = 48 on the global-memory
+    sweep in the same run, 20 cycles (counts accumulate over launches)."""
+    say("case WRITENUMFLIPS uneven batch")
+    groups = [J.make_params(L, L, nreplica=R, nconf=1, iconf0=1 + k, seed=SEED, flags=J.FLAG_WRITENUMFLIPS)
+              for k, (L, R) in enumerate([(16, 3), (24, 2), (40, 2), (48, 2)])]
+    eng, _ = run_and_compare(groups, ["hot", "str", "hot", "str"], 8, 12, flips=True, ora_configs=[0, 1, 2])
+    assert eng.sweep_path()[0] == 2
 
 
 @pytest.mark.parametrize("precision,flags,R,ptswapever,ptmeasindex,bins", [
