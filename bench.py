@@ -30,7 +30,7 @@ sys.path.insert(0, os.path.join(REPO, "j1j2-public_amd"))
 
 L, NREP, NCONF, PTSWAPEVER = 32, 16, 64, 5
 HBM_PEAK = 8.0e12                  # MI355X HBM3E, bytes/s
-TUNE_CYCLES = 40                   # the autotuner times its candidates on at most 36 cycles of one call
+TUNE_CYCLES = 60                   # the autotuner times its candidates on at most 54 cycles of one call
 GRAPH_WARM = 17                    # one 16-cycle graph launch and one single-cycle graph launch
 FP64_MIN_STEPS = 20
 SEED = 20261203

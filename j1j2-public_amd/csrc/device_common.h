@@ -212,6 +212,43 @@ __device__ __forceinline__ bool decide_f32_open(float y, uint32_t r)
 }
 
 // ---------------------------------------------------------------------------
+// The sites' random words.  A site's 32-bit word is r = v 2^16 + l: v is one
+// 16-bit half of word j of the Philox call with counter (m, c | rep << 2,
+// sweep) and l the same half of word j of the call with LO_CTR added to the
+// second counter word.  The z-layers 2 zp and 2 zp + 1 of a colour plane
+// share the calls m = zp GL + g, g the site's 4-site group within its layer
+// (GL per layer): layer 2 zp takes the high halves, layer 2 zp + 1 the low
+// halves.  The kernels draw l only when v cannot decide (decide16_filter), so
+// one call serves eight sites.
+// ---------------------------------------------------------------------------
+constexpr uint32_t LO_CTR = 0x40000000u;
+
+__host__ __device__ inline uint32_t half16(uint32_t w, int odd) { return odd ? (w & 0xFFFFu) : (w >> 16); }
+__host__ __device__ inline uint32_t site_word(uint32_t hi, uint32_t lo, int odd)
+{
+    return (half16(hi, odd) << 16) | half16(lo, odd);
+}
+
+// v as the float 2^16 + v 2^-7 (v fills the low 16 bits of the mantissa).
+__device__ __forceinline__ float v16_float(uint32_t w, int odd) { return __uint_as_float(half16(w, odd) | 0x47800000u); }
+
+// The FP32 rule's decision from v alone, for every l: with F = 2^16 + v 2^-7
+// on the float grid of spacing 2^-7, F < RN(2^16 - 2^-7 + e 2^9 (1 - 2^-16))
+// gives (v + 1) 2^16 < 2^32 (1 - 2^-16) e, so every (float)r <= (v + 1) 2^16
+// is accepted (the bound of decide_f32_filter); F > RN(2^16 + e 2^9 (1 +
+// 2^-16)) gives v 2^16 > 2^32 (1 + 2^-16) e, so every (float)r >= v 2^16 is
+// rejected (below the normal range: v >= 1, r >= 2^16 > the threshold).  Open
+// otherwise (v within about 1 + 2e of 2^16 2^y: a fraction ~2^-16 of the
+// proposals), decided by the exact rule on the whole word.
+__device__ __forceinline__ Filt decide16_filter(float y, float F)
+{
+    const float e = __builtin_amdgcn_exp2f(y);
+    const bool acc = F < __builtin_fmaf(e, 0x1.fffep8f, 0x1.fffffcp15f);
+    const bool rej = F > __builtin_fmaf(e, 0x1.0001p9f, 0x1p16f);
+    return Filt{acc, !(acc || rej)};
+}
+
+// ---------------------------------------------------------------------------
 // e^x in FP64: Cody & Waite reduction x = k ln2 + r (ln2 split so k*ln2_hi is
 // exact), Taylor series of e^r to degree 13 in Horner form, ldexp.
 // ---------------------------------------------------------------------------
@@ -239,6 +276,28 @@ __host__ __device__ inline double exp64(double x)
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
     return __builtin_ldexp(p, (int)k);
+}
+
+// The same for the FP64 rule (u = r 2^-32 < E): v accepts when (v 2^16 +
+// 2^16 - 1) 2^-32 < E and rejects when v 2^16 2^-32 >= E (both exact in FP64).
+struct Dec64 {
+    bool acc, open;
+};
+
+__device__ __forceinline__ Dec64 decide16_f64(double field, bool up, double beta, uint32_t v)
+{
+    double E;
+    if (up) {
+        if (field <= 0.0) return Dec64{true, false};
+        E = exp64(-((2.0 * field) * beta));
+    } else {
+        if (field >= 0.0) return Dec64{true, false};
+        E = exp64((2.0 * field) * beta);
+    }
+    const double u0 = (double)(v << 16) * 0x1p-32, u1 = (double)((v << 16) | 0xFFFFu) * 0x1p-32;
+    if (u1 < E) return Dec64{true, false};
+    if (u0 >= E) return Dec64{false, false};
+    return Dec64{false, true};
 }
 
 }  // namespace j1j2

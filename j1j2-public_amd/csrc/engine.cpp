@@ -683,10 +683,10 @@ std::string tune_key(const Engine &E)
 }
 
 // Times the candidates on cycles of the call (counted in `done`) when all of
-// them fit: at most TUNE_CYCLES x TUNE_REPS x (4 + 5) = 36.
-constexpr int TUNE_REPS = 2;
+// them fit: at most TUNE_CYCLES x TUNE_REPS x (4 + 5) = 54.
+constexpr int TUNE_REPS = 3;
 constexpr int TUNE_CYCLES = 2;        // cycles per timing: a lead-in and the timed one
-constexpr float TUNE_TIE = 1.01f;
+constexpr float TUNE_TIE = 1.005f;
 
 int tune(Engine &E, int measure, int ncycles, int &done)
 {

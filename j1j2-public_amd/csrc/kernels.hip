@@ -240,11 +240,24 @@ constexpr int fast_zs()
 
 // With the lattice height HC known at compile time, a colour pass of the
 // fixed-geometry stepping unrolls all of its ceil(HC / ZS) steps (at most 12).
-template <int WPH, int NT, int HC>
-constexpr bool unrolled_pass()
+// The sweep's colour passes work in z-layer pairs (one Philox call per pair
+// of words): a step covers ZP = NT / PER_Z layer pairs, threads ZP * PER_Z..
+// idle through the pass.  Used when ZP >= 1 and at least 3/4 of the threads
+// work (0 otherwise: one pair of words per thread and trip, any geometry).
+template <int WPH, int NT>
+constexpr int pair_zs()
 {
-    constexpr int ZS = fast_zs<WPH, NT>();
-    return HC > 0 && ZS >= 2 && (HC + ZS - 1) / ZS <= 12;
+    constexpr int PER_Z = 4 * WPH * WPH, ZP = NT / PER_Z;
+    return (ZP >= 1 && 4 * ZP * PER_Z >= 3 * NT) ? ZP : 0;
+}
+
+// With the lattice height HC known at compile time, a colour pass unrolls all
+// of its ceil(HC/2 / ZP) steps (at most 6, i.e. 12 words).
+template <int WPH, int NT, int HC>
+constexpr bool unrolled_pairs()
+{
+    constexpr int ZP = pair_zs<WPH, NT>();
+    return HC > 0 && ZP >= 1 && (HC / 2 + ZP - 1) / ZP <= 6;
 }
 
 template <int WPH, int NT, int HX>
@@ -335,13 +348,15 @@ __device__ __forceinline__ NbPtr nb_shift(const NbPtr &P, int d)
     return Q;
 }
 
-// Update the 4 sites of colour-plane dword `own` (plane-relative index t =
-// u*WPH + w, i.e. colour indices 4t..4t+3, Philox group t); P holds the LDS
-// addresses of the word and its neighbours.
-template <int WPH, int HX, bool SHADOW>
+// Update the 4 sites of colour-plane dword `own` in a layer of parity ODD,
+// whose random words are the ODD halves of the words `hi` of Philox call m
+// (the low 16 bits of a site's word from the call with LO_CTR, drawn only when
+// the high ones cannot decide); P holds the LDS addresses of the word and its
+// neighbours.
+template <int WPH, int HX, bool SHADOW, int ODD>
 __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int own,
-                                           int t, int w, const NbPtr &P, unsigned long long &sh_prop,
-                                           unsigned long long &sh_mis)
+                                           uint32_t m, int w, const NbPtr &P, const U4 &hi,
+                                           unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
     constexpr int HALF = 4 * WPH;
     const uint32_t S = *P.own;
@@ -356,16 +371,25 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
     const uint32_t xpe = f16_even(Xp0), xpo = f16_odd(Xp0), xme = f16_even(Xm0), xmo = f16_odd(Xm0);
     const uint32_t ype = f16_even(YP), ypo = f16_odd(YP), yme = f16_even(YM), ymo = f16_odd(YM);
-    const uint32_t ob = (uint32_t)own * 16u;            // 32-bit byte offset from a uniform base
+    // 32-bit byte offset from a uniform base, formed here: hidden from the
+    // optimiser, which would otherwise hoist every word's five 64-bit plane
+    // addresses out of the sweep loop (registers the colour passes need)
+    uint32_t ob = (uint32_t)own * 16u;
+    asm volatile("" : "+v"(ob));
     const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob), kyp = ld16(cx.kyp4, ob),
                  kym = ld16(cx.kym4, ob);
-    const U4 rn = philox4x32_10_u((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
     const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
     const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
     const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
     const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
     const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
-    const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+    const uint32_t ha[4] = {hi.x, hi.y, hi.z, hi.w};
+    uint32_t ra[4] = {0u, 0u, 0u, 0u};
+    if constexpr (SHADOW) {                             // both rules on the whole words
+        const U4 lo = philox4x32_10_u(m, cx.ctr1 | LO_CTR, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+        ra[0] = site_word(hi.x, lo.x, ODD); ra[1] = site_word(hi.y, lo.y, ODD);
+        ra[2] = site_word(hi.z, lo.z, ODD); ra[3] = site_word(hi.w, lo.w, ODD);
+    }
     uint32_t flip = 0;
     float ys[4];
     uint32_t open = 0;
@@ -388,7 +412,7 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     for (int j = 0; j < 4; ++j) {
         ys[j] = y_f32_word(h[j], S, 8 * j + 2, cx.b2);
         if constexpr (!SHADOW) {
-            const Filt f = decide_f32_filter(ys[j], ra[j]);
+            const Filt f = decide16_filter(ys[j], v16_float(ha[j], ODD));
             flip |= (f.acc ? 4u : 0u) << (8 * j);
             open |= f.open ? 1u << j : 0u;
             continue;
@@ -396,9 +420,9 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
         const bool acc = decide_f32_open(ys[j], ra[j]);
         if constexpr (SHADOW) {
             const bool up = lds_bit(S, j) != 0;
-            const int m = 4 * w + j;                          // position in the half-row
-            const int mo_p = HX ? ((m == HALF - 1) ? 0 : m + 1) : m;
-            const int mo_m = HX ? m : ((m == 0) ? HALF - 1 : m - 1);
+            const int mm = 4 * w + j;                         // position in the half-row
+            const int mo_p = HX ? ((mm == HALF - 1) ? 0 : mm + 1) : mm;
+            const int mo_m = HX ? mm : ((mm == 0) ? HALF - 1 : mm - 1);
             const double sv[10] = {pm1(lds_bit(YP, j)), pm1(lds_bit(YM, j)), pm1(lds_bit(Xp0, j)),
                                    pm1(lds_bit(Xm0, j)), pm1(lds_bit(Xp1, j)), pm1(lds_bit(Xm1, j)),
                                    pm1(lds_bit(Xp2, j)), pm1(lds_bit(Xm2, j)), pm1(lds_bit(ZP, j)),
@@ -412,102 +436,121 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
         }
         flip |= (acc ? 4u : 0u) << (8 * j);
     }
-    if (!SHADOW && open) {                                // rare: the exact rule for the open sites
+    if (!SHADOW && open) {                                // rare: the low halves and the exact rule
+        const U4 lo = philox4x32_10_u(m, cx.ctr1 | LO_CTR, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+        const uint32_t la[4] = {lo.x, lo.y, lo.z, lo.w};
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            if (((open >> j) & 1u) && decide_f32_open(ys[j], ra[j])) flip |= 4u << (8 * j);
+            if (((open >> j) & 1u) && decide_f32_open(ys[j], site_word(ha[j], la[j], ODD))) flip |= 4u << (8 * j);
     }
     *P.own = S ^ flip;
     // flip counts (WRITENUMFLIPS): this thread alone touches these four sites' counters in this pass
     if (cx.cnt_off) P.own[cx.cnt_off] += flip >> 2;
 }
 
-// One colour pass over plane c.  HC > 0: the lattice height is HC, known at
-// compile time.
-template <int WPH, int NT, int HX, bool SHADOW, int HC = 0>
-__device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
+// The odd layer of a pair from the even layer's addresses P (layer 2 zp, y
+// parity PY) and the odd layer's rows that the even layer does not read (yf:
+// plane c^2, dfa and dfb: plane c^3, at row yh + 1 for PY = 0 and yh - 1 for
+// PY = 1, all in the odd layer).  Every other neighbour word of the odd layer
+// is the even layer's word of the same role or of the other y direction, one
+// layer (PER_Z dwords) further.
+template <int PY, int PER_Z>
+__device__ __forceinline__ NbPtr odd_layer(const NbPtr &P, const uint32_t *yf, const uint32_t *dfa,
+                                           const uint32_t *dfb)
+{
+    NbPtr Q;
+    Q.own = P.own + PER_Z;
+    Q.zp = P.zp + PER_Z; Q.zm = P.zm + PER_Z; Q.xa = P.xa + PER_Z; Q.xb = P.xb + PER_Z;
+    if constexpr (PY == 0) {      // even layer: y+1 in row yh, y-1 in row yh-1; odd layer: y+1 in row yh+1, y-1 in row yh
+        Q.yp = yf; Q.ym = P.yp + PER_Z;
+        Q.d1a = dfa; Q.d1b = dfb; Q.d2a = P.d1a + PER_Z; Q.d2b = P.d1b + PER_Z;
+    } else {                      // even layer: y+1 in row yh+1, y-1 in row yh; odd layer: y+1 in row yh, y-1 in row yh-1
+        Q.yp = P.ym + PER_Z; Q.ym = yf;
+        Q.d1a = P.d2a + PER_Z; Q.d1b = P.d2b + PER_Z; Q.d2a = dfa; Q.d2b = dfb;
+    }
+    return Q;
+}
+
+// One colour pass over plane C.  HC > 0: the lattice height is HC, known at
+// compile time.  Work comes in layer pairs (2 zp, 2 zp + 1), one Philox call
+// per pair of words (m = zp PER_Z + the word's index within its layer).
+template <int WPH, int NT, int C, bool SHADOW, int HC = 0>
+__device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int H,
                                             unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
+    constexpr int HX = C & 1, PY = C >> 1;
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH;    // dwords per z-layer of a plane
-    const int nq = PER_Z * H, tid = threadIdx.x;
-    if constexpr (fast_zs<WPH, NT>() >= 2) {
-        // Each thread keeps its (y/2, w) and steps z by ZSTEP: every neighbour
-        // offset is constant except at the z boundary (for an odd ZSTEP there
-        // are two sets, P for the steps in z0's y parity and P1 for the
-        // others).  Four steps per trip: a later step's LDS addresses are an
-        // earlier one's plus a constant, which the reads take as an immediate.
-        constexpr int ZSTEP = fast_zs<WPH, NT>(), STEP = ZSTEP * PER_Z, TRIP = 4;
-        constexpr bool ODD = ZSTEP & 1;
-        if (tid >= STEP) return;
-        const int tz = tid % PER_Z, w = tz % WPH;
-        int z0 = tid / PER_Z;
-        if constexpr (PER_Z % 64 == 0) z0 = __builtin_amdgcn_readfirstlane(z0);   // wave-uniform
-        const int u0 = z0 * HALF + tz / WPH;
-        const UnitGeo g0 = unit_geo<WPH>(u0, c, H);
-        NbOff o = nb_offsets<WPH, HX>(c, nq, u0, w, g0);
-        const int dY = ((c ^ 2) - c) * nq;
-        const int nk = (H - z0 + ZSTEP - 1) / ZSTEP;
-        const int own0 = c * nq + z0 * PER_Z + tz;
+    const int nq = PER_Z * H, tid = threadIdx.x, npairs = H >> 1;
+    if constexpr (pair_zs<WPH, NT>() >= 1) {
+        // Each thread keeps its (y/2, w) and steps over layer pairs by ZP
+        // pairs: every neighbour offset is constant except at the z boundary
+        // (layer 0, the even layer of the first pair, and layer H - 1, the odd
+        // layer of the last), so a later step's LDS addresses are an earlier
+        // one's plus a constant, which the reads take as an immediate.
+        constexpr int ZP = pair_zs<WPH, NT>(), STEP = 2 * ZP * PER_Z, TRIP = 2;
+        if (tid >= ZP * PER_Z) return;
+        const int tz = tid % PER_Z, w = tz % WPH, yh = tz / WPH;
+        int zp0 = tid / PER_Z;
+        if constexpr (PER_Z % 64 == 0) zp0 = __builtin_amdgcn_readfirstlane(zp0);   // wave-uniform
+        const int u0 = 2 * zp0 * HALF + yh;
+        NbOff o = nb_offsets<WPH, HX>(C, nq, u0, w, unit_geo<WPH>(u0, C, H));
+        const int dY = ((C ^ 2) - C) * nq, dD = ((C ^ 3) - C) * nq;
         o.zp = dY + PER_Z;                                // away from the z boundary
         o.zm = dY - PER_Z;
-        NbPtr P = nb_ptrs(lds32, own0, o), P1 = P;
+        const int own0 = C * nq + 2 * zp0 * PER_Z + tz;
+        NbPtr P = nb_ptrs(lds32, own0, o);
+        // the odd layer's far row (unit offset from the thread's unit, periodic in y)
+        const int far = PY ? (yh == 0 ? HALF - 1 : -1) : (yh == HALF - 1 ? -(HALF - 1) : 1);
+        const uint32_t *yf = P.own + PER_Z + dY + far * WPH;
+        const uint32_t *dfa = P.own + PER_Z + dD + far * WPH;
+        const uint32_t *dfb = dfa + (o.xb - o.xa);
         // materialise each address in one VGPR (otherwise the plane base is
         // re-added to every address on every trip)
         opaque(P.own); opaque(P.yp); opaque(P.ym); opaque(P.zp); opaque(P.zm); opaque(P.xa); opaque(P.xb);
-        opaque(P.d1a); opaque(P.d1b); opaque(P.d2a); opaque(P.d2b);
-        if constexpr (ODD) {
-            // the second step's unit (z0 + ZSTEP, the other y parity); only
-            // dereferenced when that step exists
-            const int u1 = u0 + ZSTEP * HALF;
-            NbOff o1 = nb_offsets<WPH, HX>(c, nq, u1, w, unit_geo<WPH>(u1, c, H));
-            o1.zp = dY + PER_Z;
-            o1.zm = dY - PER_Z;
-            P1 = nb_ptrs(lds32, own0 + STEP, o1);
-            opaque(P1.own); opaque(P1.yp); opaque(P1.ym); opaque(P1.zp); opaque(P1.zm); opaque(P1.xa);
-            opaque(P1.xb); opaque(P1.d1a); opaque(P1.d1b); opaque(P1.d2a); opaque(P1.d2b);
-        }
-        if constexpr (unrolled_pass<WPH, NT, HC>()) {
-            // a thread (first layer z0 < ZSTEP) takes NK = ceil(HC / ZSTEP)
-            // steps, the last one only if z0 + (NK-1) ZSTEP < HC: all of them
-            // unrolled, so every step's LDS addresses are the first step's
-            // plus an immediate.  Layer HC-1 is some thread's last step and
-            // layer 0 a first step, so only those meet the z boundary.
-            constexpr int NK = (HC + ZSTEP - 1) / ZSTEP;
-            constexpr bool RAGGED = HC % ZSTEP != 0;
+        opaque(P.d1a); opaque(P.d1b); opaque(P.d2a); opaque(P.d2b); opaque(yf); opaque(dfa); opaque(dfb);
+        const uint32_t m0 = (uint32_t)(zp0 * PER_Z + tz);
+        const auto step = [&](int kk, int zp) {
+            const uint32_t m = m0 + (uint32_t)(kk * ZP * PER_Z);
+            const U4 hi = philox4x32_10_u(m, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+            NbPtr Q = nb_shift(P, kk * STEP);
+            if (zp == 0) Q.zm += H * PER_Z;                // periodic in z
+            update_word<WPH, HX, SHADOW, 0>(lds32, cx, cc, own0 + kk * STEP, m, w, Q, hi, sh_prop, sh_mis);
+            NbPtr Q1 = nb_shift(odd_layer<PY, PER_Z>(P, yf, dfa, dfb), kk * STEP);
+            if (zp == npairs - 1) Q1.zp -= H * PER_Z;
+            update_word<WPH, HX, SHADOW, 1>(lds32, cx, cc, own0 + kk * STEP + PER_Z, m, w, Q1, hi, sh_prop,
+                                            sh_mis);
+        };
+        if constexpr (unrolled_pairs<WPH, NT, HC>()) {
+            // a thread (first pair zp0 < ZP) takes NK = ceil(HC/2 / ZP) steps,
+            // the last one only if zp0 + (NK-1) ZP < HC/2: all of them unrolled
+            constexpr int NK = (HC / 2 + ZP - 1) / ZP;
+            constexpr bool RAGGED = (HC / 2) % ZP != 0;
 #pragma unroll
             for (int kk = 0; kk < NK; ++kk) {
-                const int z = z0 + kk * ZSTEP;
-                if (RAGGED && kk == NK - 1 && z >= HC) break;
-                NbPtr Q = (ODD && (kk & 1)) ? nb_shift(P1, (kk - 1) * STEP) : nb_shift(P, kk * STEP);
-                if (kk == NK - 1 && z == HC - 1) Q.zp -= HC * PER_Z;
-                if (kk == 0 && z == 0) Q.zm += HC * PER_Z;
-                update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q, sh_prop, sh_mis);
+                const int zp = zp0 + kk * ZP;
+                if (RAGGED && kk == NK - 1 && zp >= HC / 2) break;
+                step(kk, zp);
             }
         } else {
+            const int nk = (npairs - zp0 + ZP - 1) / ZP;
             for (int k = 0; k < nk; k += TRIP) {
 #pragma unroll
-                for (int i = 0; i < TRIP; ++i) {
-                    const int kk = k + i;
-                    if (kk < nk) {
-                        const int z = z0 + kk * ZSTEP;
-                        NbPtr Q = (ODD && (i & 1)) ? nb_shift(P1, (i - 1) * STEP) : nb_shift(P, i * STEP);
-                        if (z == H - 1) Q.zp -= H * PER_Z;     // periodic in z
-                        if (z == 0) Q.zm += H * PER_Z;
-                        update_word<WPH, HX, SHADOW>(lds32, cx, cc, own0 + kk * STEP, z * PER_Z + tz, w, Q,
-                                                     sh_prop, sh_mis);
-                    }
-                }
-                P = nb_shift(P, TRIP * STEP);
-                if constexpr (ODD) P1 = nb_shift(P1, TRIP * STEP);
+                for (int i = 0; i < TRIP; ++i)
+                    if (k + i < nk) step(k + i, zp0 + (k + i) * ZP);
             }
         }
     } else {
-        for (int t = tid; t < nq; t += NT) {
-            const int u = t / WPH, w = t - u * WPH;
-            const UnitGeo g = unit_geo<WPH>(u, c, H);
-            const NbOff o = nb_offsets<WPH, HX>(c, nq, u, w, g);
-            update_word<WPH, HX, SHADOW>(lds32, cx, cc, c * nq + t, t, w, nb_ptrs(lds32, c * nq + t, o), sh_prop,
-                                         sh_mis);
+        for (int mm = tid; mm < nq / 2; mm += NT) {
+            const int zp = mm / PER_Z, tz = mm - zp * PER_Z;
+            const U4 hi = philox4x32_10_u((uint32_t)mm, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+            const int t0 = 2 * zp * PER_Z + tz, t1 = t0 + PER_Z;
+            const int u0 = t0 / WPH, w = t0 - u0 * WPH, u1 = u0 + HALF;
+            const NbOff o0 = nb_offsets<WPH, HX>(C, nq, u0, w, unit_geo<WPH>(u0, C, H));
+            update_word<WPH, HX, SHADOW, 0>(lds32, cx, cc, C * nq + t0, (uint32_t)mm, w,
+                                            nb_ptrs(lds32, C * nq + t0, o0), hi, sh_prop, sh_mis);
+            const NbOff o1 = nb_offsets<WPH, HX>(C, nq, u1, w, unit_geo<WPH>(u1, C, H));
+            update_word<WPH, HX, SHADOW, 1>(lds32, cx, cc, C * nq + t1, (uint32_t)mm, w,
+                                            nb_ptrs(lds32, C * nq + t1, o1), hi, sh_prop, sh_mis);
         }
     }
 }
@@ -583,8 +626,12 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 #pragma unroll 1
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
-            if (c & 1) colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, c, H, sh_prop, sh_mis);
-            else colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, c, H, sh_prop, sh_mis);
+            switch (c) {
+                case 0: colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
+                case 1: colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
+                case 2: colour_pass<WPH, NT, 2, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
+                default: colour_pass<WPH, NT, 3, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
+            }
             __syncthreads();
         }
     }
@@ -661,12 +708,26 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 // balances the XCDs by work and orders each XCD's lattices largest first.
 
 
-// One geometry.  At most 128 VGPRs (four waves per SIMD): e.g. four 256-thread
-// blocks, i.e. four L = 32 lattices (32 KiB of LDS each), or one 1024-thread
-// block stay resident on every CU.  HC > 0: every lattice of the launch has
-// height HC (cubic lattices, H = L).
+// Waves per SIMD the resident kernel is compiled for (its register budget:
+// 512 / waves VGPRs): for cubic lattices, the count at which one more block
+// of NT threads fits on a CU when the lattices' LDS would allow it -- 512
+// threads at L <= 32: 6 (three blocks; four L = 32 lattices fit the LDS), 576
+// threads at L <= 40: 5 (two blocks), 256 threads at L <= 24: 6; otherwise 4
+// (128 VGPRs).
+template <int WPH, int NT, int HC>
+constexpr int res_waves()
+{
+    if (HC == 0) return 4;
+    if (NT == 512 && WPH <= 4) return 6;
+    if (NT == 576 && WPH <= 5) return 5;
+    if (NT == 256 && WPH <= 3) return 6;
+    return 4;
+}
+
+// One geometry.  HC > 0: every lattice of the launch has height HC (cubic
+// lattices, H = L).
 template <int WPH, int NT, bool SHADOW, int HC = 0>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : res_waves<WPH, NT, HC>(), 8)))
 void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
 {
     // the lattice at LDS address 0 (so neighbour addresses need no base), the
@@ -703,7 +764,8 @@ void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
 
 // ---------------------------------------------------------------------------
 // Global-memory sweeps (any even L): one colour pass per launch, one thread
-// per Philox group of 4 same-colour sites, neighbours read from L2/HBM.
+// per Philox call (two 4-site groups, one in each layer of a pair),
+// neighbours read from L2/HBM.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void colour_site(int L, int c, int q, int &x, int &y, int &z)
 {
@@ -734,24 +796,45 @@ __device__ __forceinline__ Nbr neighbours(int L, int H, int x, int y, int z)
     return n;
 }
 
+// Philox calls of one colour plane: a call per 4-site group of a layer pair
+// (GL = ceil((L/2)^2 / 4) groups per layer; the last one of a layer is short
+// when (L/2)^2 % 4 != 0).
+__host__ __device__ inline int global_calls(int L, int H)
+{
+    const int half = L >> 1;
+    return (H >> 1) * ((half * half + 3) / 4);
+}
+
 template <int PREC, bool SHADOW>   // PREC 1 = FP32 rule, 0 = FP64 plain rule
 __global__ __launch_bounds__(GLB_THREADS) void k_sweep_global(DeviceState st, int32_t lat_begin,
                                                               int32_t L, int32_t H, int32_t k, int32_t c)
 {
     const int lat = lat_begin + blockIdx.y;
     const LatDesc d = st.lats[lat];
-    const int ncol = (L * L * H) / 4;
-    const int g = blockIdx.x * GLB_THREADS + threadIdx.x;
-    if (4 * g >= ncol) return;
+    const int half = L >> 1, lay = half * half, GL = (lay + 3) / 4;
+    const int m = blockIdx.x * GLB_THREADS + threadIdx.x;
+    if (m >= (H >> 1) * GL) return;
+    const int zp = m / GL, gl = m - zp * GL;
     const ConfDesc cd = st.confs[d.conf];
     const ConfConst cc = st.cconst[d.conf];
     const int slot = st.temp_of_rep[d.slot_base + d.rep];
     const float b2 = st.b2[d.slot_base + slot];
     const double beta = st.beta[d.slot_base + slot];
     const uint64_t sweep = clock_cycle(st) * (uint64_t)st.ptswapever + (uint64_t)k;
-    const U4 rn = philox4x32_10((uint32_t)g, (uint32_t)c | ((uint32_t)d.rep << 2), (uint32_t)sweep,
-                                (uint32_t)(sweep >> 32), st.seed, (uint32_t)cd.iconf);
-    const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+    const uint32_t ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
+    const U4 rn = philox4x32_10((uint32_t)m, ctr1, (uint32_t)sweep, (uint32_t)(sweep >> 32), st.seed,
+                                (uint32_t)cd.iconf);
+    const uint32_t ha[4] = {rn.x, rn.y, rn.z, rn.w};
+    uint32_t la[4] = {0u, 0u, 0u, 0u};
+    bool have_lo = false;
+    const auto draw_lo = [&]() {      // the low halves, drawn once and only when needed
+        if (have_lo) return;
+        const U4 lo = philox4x32_10((uint32_t)m, ctr1 | LO_CTR, (uint32_t)sweep, (uint32_t)(sweep >> 32), st.seed,
+                                    (uint32_t)cd.iconf);
+        la[0] = lo.x; la[1] = lo.y; la[2] = lo.z; la[3] = lo.w;
+        have_lo = true;
+    };
+    if (SHADOW) draw_lo();
     uint8_t *sp = st.spins + d.spin_off;
     const int64_t so = d.site_off;
     const float *kc = st.kd + 0 * st.nsites + so, *kxp = st.kd + 1 * st.nsites + so,
@@ -760,43 +843,66 @@ __global__ __launch_bounds__(GLB_THREADS) void k_sweep_global(DeviceState st, in
     const double *p = st.p64 + so;
     const int N = L * L * H;
     unsigned long long sh_prop = 0, sh_mis = 0;
-    for (int j = 0; j < 4; ++j) {
-        const int q = 4 * g + j;
-        if (q >= ncol) break;
-        int x, y, z;
-        colour_site(L, c, q, x, y, z);
-        const Nbr nb = neighbours(L, H, x, y, z);
-        uint32_t b[10];
+    for (int odd = 0; odd < 2; ++odd)
+        for (int j = 0; j < 4; ++j) {
+            const int qz = 4 * gl + j;
+            if (qz >= lay) break;
+            int x, y, z;
+            colour_site(L, c, (2 * zp + odd) * lay + qz, x, y, z);
+            const Nbr nb = neighbours(L, H, x, y, z);
+            uint32_t b[10];
 #pragma unroll
-        for (int t = 0; t < 10; ++t) b[t] = sp[nb.m[t]];
-        const uint32_t up = sp[nb.i];
-        bool acc, other = false;
-        if (PREC == 1 || SHADOW) {
-            // planes hold 2^14 * 2K: (2^-14 b) (2^14 2K) is exactly b 2K
-            float h = kc[nb.i];
-            h = __builtin_fmaf((float)b[2] * 0x1p-14f, kxp[nb.i], h);
-            h = __builtin_fmaf((float)b[3] * 0x1p-14f, kxm[nb.i], h);
-            h = __builtin_fmaf((float)b[0] * 0x1p-14f, kyp[nb.i], h);
-            h = __builtin_fmaf((float)b[1] * 0x1p-14f, kym[nb.i], h);
-            h = __builtin_fmaf((float)(b[4] + b[5] + b[6] + b[7]), cc.j2x2, h);
-            h = __builtin_fmaf((float)(b[8] + b[9]), cc.j3x2, h);
-            const bool a32 = decide_f32_word(h, up, 0, b2, ra[j]);
-            if (PREC == 1) acc = a32; else other = a32;
-        }
-        if (PREC == 0 || SHADOW) {
-            double sv[10];
+            for (int t = 0; t < 10; ++t) b[t] = sp[nb.m[t]];
+            const uint32_t up = sp[nb.i];
+            bool acc, other = false;
+            if (PREC == 1 || SHADOW) {
+                // planes hold 2^14 * 2K: (2^-14 b) (2^14 2K) is exactly b 2K
+                float h = kc[nb.i];
+                h = __builtin_fmaf((float)b[2] * 0x1p-14f, kxp[nb.i], h);
+                h = __builtin_fmaf((float)b[3] * 0x1p-14f, kxm[nb.i], h);
+                h = __builtin_fmaf((float)b[0] * 0x1p-14f, kyp[nb.i], h);
+                h = __builtin_fmaf((float)b[1] * 0x1p-14f, kym[nb.i], h);
+                h = __builtin_fmaf((float)(b[4] + b[5] + b[6] + b[7]), cc.j2x2, h);
+                h = __builtin_fmaf((float)(b[8] + b[9]), cc.j3x2, h);
+                const float yv = y_f32_word(h, up, 0, b2);
+                bool a32;
+                if (SHADOW) {
+                    a32 = decide_f32_open(yv, site_word(ha[j], la[j], odd));
+                } else {
+                    const Filt f = decide16_filter(yv, v16_float(ha[j], odd));
+                    a32 = f.acc;
+                    if (f.open) {
+                        draw_lo();
+                        a32 = decide_f32_open(yv, site_word(ha[j], la[j], odd));
+                    }
+                }
+                if (PREC == 1) acc = a32; else other = a32;
+            }
+            if (PREC == 0 || SHADOW) {
+                double sv[10];
 #pragma unroll
-            for (int t = 0; t < 10; ++t) sv[t] = pm1(b[t]);
-            const double pv[5] = {p[nb.i], p[nb.m[0]], p[nb.m[1]], p[nb.m[2]], p[nb.m[3]]};
-            const bool a64 = decide_f64(field_f64(sv, pv, cc), up != 0, beta, ra[j]);
-            if (PREC == 0) acc = a64; else other = a64;
+                for (int t = 0; t < 10; ++t) sv[t] = pm1(b[t]);
+                const double pv[5] = {p[nb.i], p[nb.m[0]], p[nb.m[1]], p[nb.m[2]], p[nb.m[3]]};
+                const double field = field_f64(sv, pv, cc);
+                bool a64;
+                if (SHADOW) {
+                    a64 = decide_f64(field, up != 0, beta, site_word(ha[j], la[j], odd));
+                } else {
+                    const Dec64 dd = decide16_f64(field, up != 0, beta, half16(ha[j], odd));
+                    a64 = dd.acc;
+                    if (dd.open) {
+                        draw_lo();
+                        a64 = decide_f64(field, up != 0, beta, site_word(ha[j], la[j], odd));
+                    }
+                }
+                if (PREC == 0) acc = a64; else other = a64;
+            }
+            if (SHADOW) { sh_prop += 1; sh_mis += (other != acc); }
+            if (acc) {
+                sp[nb.i] = (uint8_t)(up ^ 1u);
+                if (st.flips) st.flips[cd.flip_off + (int64_t)slot * N + nb.i] += 1;
+            }
         }
-        if (SHADOW) { sh_prop += 1; sh_mis += (other != acc); }
-        if (acc) {
-            sp[nb.i] = (uint8_t)(up ^ 1u);
-            if (st.flips) st.flips[cd.flip_off + (int64_t)slot * N + nb.i] += 1;
-        }
-    }
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
         sh_mis = wave_sum<unsigned long long>(sh_mis);
@@ -1153,8 +1259,8 @@ hipError_t launch_sweep_global(const DeviceState &st, int32_t lat_begin, int32_t
                                int32_t k, int32_t colour, bool shadow, hipStream_t s)
 {
     if (nlat <= 0) return hipSuccess;
-    const int ngroups = ((L * L * H) / 4 + 3) / 4;
-    const dim3 grid((ngroups + GLB_THREADS - 1) / GLB_THREADS, nlat), block(GLB_THREADS);
+    const int ncalls = global_calls(L, H);
+    const dim3 grid((ncalls + GLB_THREADS - 1) / GLB_THREADS, nlat), block(GLB_THREADS);
     if (shadow)
         hipLaunchKernelGGL((k_sweep_global<1, true>), grid, block, 0, s, st, lat_begin, L, H, k, colour);
     else
@@ -1166,8 +1272,8 @@ hipError_t launch_sweep_plain64(const DeviceState &st, int32_t lat_begin, int32_
                                 int32_t k, int32_t colour, hipStream_t s)
 {
     if (nlat <= 0) return hipSuccess;
-    const int ngroups = ((L * L * H) / 4 + 3) / 4;
-    const dim3 grid((ngroups + GLB_THREADS - 1) / GLB_THREADS, nlat), block(GLB_THREADS);
+    const int ncalls = global_calls(L, H);
+    const dim3 grid((ncalls + GLB_THREADS - 1) / GLB_THREADS, nlat), block(GLB_THREADS);
     hipLaunchKernelGGL((k_sweep_global<0, false>), grid, block, 0, s, st, lat_begin, L, H, k, colour);
     return hipGetLastError();
 }

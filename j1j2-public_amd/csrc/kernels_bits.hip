@@ -35,7 +35,7 @@ namespace {
 template <int L>
 struct BitGeo {
     static constexpr int HW = L / 2;                 // sites per half-row
-    static constexpr int G = HW / 4;                 // 4-site groups per half-row (= Philox groups)
+    static constexpr int G = HW / 4;                 // 4-site groups per half-row
     static constexpr int TPL = G * HW;               // groups per z-layer of a plane: one thread each
     static constexpr int NT = (TPL + 63) / 64 * 64;  // threads per block (idle tail lanes)
     static constexpr int G0 = (G + 1) / 2;           // groups in dword 0 of a unit
@@ -162,12 +162,14 @@ struct Share {
 // One 4-site group of a colour pass of the bit layout: thread (yh, j) owns
 // group j of the half-row (zg, yh); its flips go to the unit word with an LDS
 // atomic xor (the G threads of a unit share the word; nobody reads plane c in
-// pass c).  Philox counters and disorder planes are indexed by the global
+// pass c).  Its random words are the ODD (= zg & 1) halves of the words `hi`
+// of Philox call m = (zg / 2) TPL + tid (low halves from the call with LO_CTR
+// when needed).  Philox counters and disorder planes are indexed by the global
 // layer zg, so a slab updates its sites exactly as the whole lattice would.
-template <int L, int HX, bool SHADOW, bool SLAB>
+template <int L, int HX, bool SHADOW, bool SLAB, int ODD>
 __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
-                                          int NU, int zg, int zl, int yh, int j, unsigned long long &sh_prop,
-                                          unsigned long long &sh_mis)
+                                          int NU, int zg, int zl, int yh, int j, uint32_t m, const U4 &hi,
+                                          unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
     using Gm = BitGeo<L>;
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
@@ -181,18 +183,23 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
     const uint32_t xpe = f16_even(s.xp), xpo = f16_odd(s.xp), xme = f16_even(s.xm), xmo = f16_odd(s.xm);
     const uint32_t ype = f16_even(s.yp), ypo = f16_odd(s.yp), yme = f16_even(s.ym), ymo = f16_odd(s.ym);
-    const int t = zg * TPL + tid;                             // plane-relative group = Philox counter
-    const int own = c * nq + t;
-    const uint32_t ob = (uint32_t)own * 16u;
+    const int own = c * nq + zg * TPL + tid;                  // the group's dword in the colour-plane order
+    uint32_t ob = (uint32_t)own * 16u;                        // formed here, not hoisted out of the sweep loop
+    asm volatile("" : "+v"(ob));
     const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob),
                  kyp = ld16(cx.kyp4, ob), kym = ld16(cx.kym4, ob);
-    const U4 rn = philox4x32_10_u((uint32_t)t, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
     const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
     const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
     const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
     const float kypa[4] = {kyp.x, kyp.y, kyp.z, kyp.w};
     const float kyma[4] = {kym.x, kym.y, kym.z, kym.w};
-    const uint32_t ra[4] = {rn.x, rn.y, rn.z, rn.w};
+    const uint32_t ha[4] = {hi.x, hi.y, hi.z, hi.w};
+    uint32_t ra[4] = {0u, 0u, 0u, 0u};
+    if constexpr (SHADOW) {                                   // both rules on the whole words
+        const U4 lo = philox4x32_10_u(m, cx.ctr1 | LO_CTR, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+        ra[0] = site_word(hi.x, lo.x, ODD); ra[1] = site_word(hi.y, lo.y, ODD);
+        ra[2] = site_word(hi.z, lo.z, ODD); ra[3] = site_word(hi.w, lo.w, ODD);
+    }
     uint32_t f = 0, open = 0;
     float ys[4], h[4];
     // the four sites' fields term by term (each site's sum in the same order):
@@ -213,7 +220,7 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     for (int k = 0; k < 4; ++k) {
         ys[k] = y_f32_word(h[k], s.S, k, cx.b2);
         if constexpr (!SHADOW) {
-            const Filt fl = decide_f32_filter(ys[k], ra[k]);
+            const Filt fl = decide16_filter(ys[k], v16_float(ha[k], ODD));
             f |= (fl.acc ? 1u : 0u) << k;
             open |= fl.open ? 1u << k : 0u;
             continue;
@@ -241,10 +248,12 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
         }
         f |= (acc ? 1u : 0u) << k;
     }
-    if (!SHADOW && open) {                                // rare: the exact rule for the open sites
+    if (!SHADOW && open) {                                // rare: the low halves and the exact rule
+        const U4 lo = philox4x32_10_u(m, cx.ctr1 | LO_CTR, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+        const uint32_t la[4] = {lo.x, lo.y, lo.z, lo.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (((open >> k) & 1u) && decide_f32_open(ys[k], ra[k])) f |= 1u << k;
+            if (((open >> k) & 1u) && decide_f32_open(ys[k], site_word(ha[k], la[k], ODD))) f |= 1u << k;
     }
     // flips into the group's dword; the first and last group of each dword
     // also flip their site's copy in the other dword of the unit
@@ -258,10 +267,9 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     if (cm) atomicXor(uw + (1 - gp.h), cm);
 }
 
-// One colour pass over the block's layers.  Whole lattice: two layers per
-// trip (the two groups are independent, which the scheduler interleaves; nz
-// is even).  Slab: one per trip, within the register budget of three blocks
-// per CU, whose waves hide the latency instead.
+// One colour pass over the block's layers, a layer pair per trip (one Philox
+// call for the two groups; the block's first layer and its layer count are
+// even).
 template <int L, int HX, bool SHADOW, bool SLAB>
 __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
                                          const Share &sh, unsigned long long &sh_prop, unsigned long long &sh_mis)
@@ -271,14 +279,12 @@ __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, co
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G;
-    if constexpr (SLAB) {
-        for (int zl = 0; zl < sh.nz; ++zl)
-            bit_group<L, HX, SHADOW, SLAB>(lds32, cx, cc, c, H, sh.NU, sh.z0 + zl, zl, yh, j, sh_prop, sh_mis);
-    } else {
-        for (int zl = 0; zl < sh.nz; zl += 2) {
-            bit_group<L, HX, SHADOW, SLAB>(lds32, cx, cc, c, H, sh.NU, zl, zl, yh, j, sh_prop, sh_mis);
-            bit_group<L, HX, SHADOW, SLAB>(lds32, cx, cc, c, H, sh.NU, zl + 1, zl + 1, yh, j, sh_prop, sh_mis);
-        }
+    for (int zl = 0; zl < sh.nz; zl += 2) {
+        const int zg = sh.z0 + zl;
+        const uint32_t m = (uint32_t)((zg >> 1) * TPL + tid);
+        const U4 hi = philox4x32_10_u(m, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
+        bit_group<L, HX, SHADOW, SLAB, 0>(lds32, cx, cc, c, H, sh.NU, zg, zl, yh, j, m, hi, sh_prop, sh_mis);
+        bit_group<L, HX, SHADOW, SLAB, 1>(lds32, cx, cc, c, H, sh.NU, zg + 1, zl + 1, yh, j, m, hi, sh_prop, sh_mis);
     }
 }
 

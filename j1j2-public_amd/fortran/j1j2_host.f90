@@ -345,7 +345,7 @@ contains
       integer(c_int), intent(in) :: phase
       character(len=*), intent(in) :: label
       integer, intent(in) :: iconf, total
-      integer, parameter :: TUNE_CYCLES = 40
+      integer, parameter :: TUNE_CYCLES = 60
       integer :: every, done, chunk
       every = max(1, total/10)
       call write_progress(label, iconf, 0, total)

@@ -318,34 +318,69 @@ static int decide_f64(const ora_lattice *lat, const int8_t *s, int32_t site, dou
     }
 }
 
+/* The engine's random words (no counterpart in the reference, whose
+ * Metro_sweep draws KISS05 uniforms in site order).  Colour c of replica
+ * `replica` in sweep `sweep`: the z-layers 2zp and 2zp+1 share the Philox
+ * calls m = zp*GL + g (g = the site's 4-site group within its layer, GL =
+ * ceil((L/2)^2 / 4) groups per layer); the site's word is r = v*2^16 + l,
+ * v = one 16-bit half of word j of call (m, c | replica<<2, sweep lo, sweep
+ * hi), l = the same half of word j of the call with ORA_LO_CTR added to the
+ * second counter word; the even layer takes the high halves, the odd layer
+ * the low halves.  out[] receives r for the colour indices qi = z*(L/2)^2 +
+ * 4g + j of the call's sites (up to 8: j = 0..3 of the even layer, then of
+ * the odd layer; n_out of them exist). */
+#define ORA_LO_CTR 0x40000000u
+
+static uint32_t half16(uint32_t w, int odd) { return odd ? (w & 0xFFFFu) : (w >> 16); }
+
+int ora_call_words(int32_t L, uint32_t seed, uint32_t iconf, uint32_t replica, uint64_t sweep, int32_t c,
+                   int32_t m, int32_t qi_out[8], uint32_t r_out[8])
+{
+    int32_t half = L / 2, lay = half * half, GL = (lay + 3) / 4;
+    int32_t zp = m / GL, g = m % GL, n = 0;
+    uint32_t key[2] = { seed, iconf };
+    uint32_t c1 = (uint32_t)c | (replica << 2);
+    uint32_t ch[4] = { (uint32_t)m, c1, (uint32_t)sweep, (uint32_t)(sweep >> 32) };
+    uint32_t cl[4] = { (uint32_t)m, c1 | ORA_LO_CTR, (uint32_t)sweep, (uint32_t)(sweep >> 32) };
+    uint32_t hi[4], lo[4];
+    ora_philox4x32_10(ch, key, hi);
+    ora_philox4x32_10(cl, key, lo);
+    for (int odd = 0; odd < 2; ++odd)
+        for (int j = 0; j < 4; ++j) {
+            int32_t qz = 4 * g + j;
+            if (qz >= lay) break;
+            qi_out[n] = (2 * zp + odd) * lay + qz;
+            r_out[n] = (half16(hi[j], odd) << 16) | half16(lo[j], odd);
+            ++n;
+        }
+    return n;
+}
+
 void ora_sweep(const ora_lattice *lat, int8_t *s, int precision, double beta,
                uint32_t seed, uint32_t iconf, uint32_t replica, uint64_t sweep,
                int32_t *flipcounts, int64_t *shadow)
 {
     int32_t L = lat->L, H = lat->H, half = L / 2, L2 = L * L;
-    int32_t ncol = lat->N / 4;
+    int32_t ncalls = (H / 2) * ((half * half + 3) / 4);
     float b2 = (float)(2.0 * beta * 0x1.71547652b82fep+0);
-    uint32_t key[2] = { seed, iconf };
     for (int32_t c = 0; c < 4; ++c) {
-        for (int32_t g = 0; g < (ncol + 3) / 4; ++g) {
-            uint32_t ctr[4] = { (uint32_t)g, (uint32_t)c | (replica << 2),
-                                (uint32_t)sweep, (uint32_t)(sweep >> 32) };
-            uint32_t r4[4];
-            ora_philox4x32_10(ctr, key, r4);
-            for (int j = 0; j < 4; ++j) {
-                int32_t qi = 4 * g + j;
-                if (qi >= ncol) break;
+        for (int32_t m = 0; m < ncalls; ++m) {
+            int32_t qis[8];
+            uint32_t rs[8];
+            int n = ora_call_words(L, seed, iconf, replica, sweep, c, m, qis, rs);
+            for (int k = 0; k < n; ++k) {
+                int32_t qi = qis[k];
                 int32_t xh = qi % half, rr = qi / half;
                 int32_t z = rr / half, yh = rr % half;
                 int32_t x = 2 * xh + (c & 1);
                 int32_t y = 2 * yh + (((c >> 1) ^ z) & 1);
                 int32_t site = x + L * y + L2 * z;
                 int acc;
-                if (precision == 0) acc = decide_f64(lat, s, site, beta, r4[j]);
-                else acc = decide_f32(lat, s, site, b2, r4[j]);
+                if (precision == 0) acc = decide_f64(lat, s, site, beta, rs[k]);
+                else acc = decide_f32(lat, s, site, b2, rs[k]);
                 if (shadow) {
-                    int other = (precision == 0) ? decide_f32(lat, s, site, b2, r4[j])
-                                                 : decide_f64(lat, s, site, beta, r4[j]);
+                    int other = (precision == 0) ? decide_f32(lat, s, site, b2, rs[k])
+                                                 : decide_f64(lat, s, site, beta, rs[k]);
                     shadow[0] += 1;
                     shadow[1] += (other != acc);
                 }
@@ -355,7 +390,6 @@ void ora_sweep(const ora_lattice *lat, int8_t *s, int precision, double beta,
                 }
             }
         }
-        (void)H;
     }
 }
 

@@ -23,6 +23,9 @@
  *   initial spins -> the reference's cfen of HOT and STR starts run with
  *                   NEQ = NMESS = 0
  *   philox       -> published Random123 known-answer vectors
+ *   site words   -> every site of a colour gets exactly one call's word
+ *                   half, each half of each call used once, high and low
+ *                   parts from different calls (test_oracle.py)
  *   exp64, threshold32 -> exp / 2^y in double over dense grids
  *   fields       -> brute-force dE = E(after) - E(before); closed forms of
  *                   the stripe and ferro states
@@ -81,6 +84,11 @@ float  ora_field_f32(const ora_lattice *lat, const int8_t *s, int32_t site);
 /* Colour of a site and its index within the colour (engine order). */
 int32_t ora_colour(int32_t L, int32_t site);
 int32_t ora_colour_index(int32_t L, int32_t site);
+
+/* The sites of Philox call m of colour c (engine stream; see the .c file):
+ * their colour indices and 32-bit words, up to 8; returns how many. */
+int ora_call_words(int32_t L, uint32_t seed, uint32_t iconf, uint32_t replica, uint64_t sweep, int32_t c,
+                   int32_t m, int32_t qi_out[8], uint32_t r_out[8]);
 
 /* One 4-colour sweep of one replica.  precision: 0 = FP64 (reference field
  * order), 1 = FP32 engine rule.  beta is FP64; the FP32 form uses

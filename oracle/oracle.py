@@ -65,6 +65,9 @@ def _declare(L):
     L.ora_kiss_gauss.restype = _d
     L.ora_kiss_fill.argtypes = [_i32, _i64, _P]
     L.ora_philox4x32_10.argtypes = [_P, _P, _P]
+    L.ora_call_words.argtypes = [ctypes.c_int32, _u32, _u32, _u32, ctypes.c_uint64, ctypes.c_int32,
+                                 ctypes.c_int32, _P, _P]
+    L.ora_call_words.restype = ctypes.c_int
     L.ora_accept_threshold_f32.argtypes = [_f]
     L.ora_accept_threshold_f32.restype = _f
     L.ora_exp64.argtypes = [_d]
@@ -126,6 +129,14 @@ def philox(ctr, key):
     o = np.zeros(4, np.uint32)
     lib().ora_philox4x32_10(_ptr(c), _ptr(k), _ptr(o))
     return o
+
+
+def call_words(L, seed, iconf, replica, sweep, c, m):
+    """Colour indices and 32-bit words of the sites of Philox call m."""
+    qi = np.zeros(8, np.int32)
+    r = np.zeros(8, np.uint32)
+    n = lib().ora_call_words(L, seed, iconf, replica, sweep, c, m, _ptr(qi), _ptr(r))
+    return qi[:n], r[:n]
 
 
 def accept_threshold_f32(y):

@@ -113,7 +113,7 @@ def test_uneven_full_batch_autotuned_is_bit_identical(monkeypatch):
     res = []
     for tune in (True, False):
         eng, _ = c4_engine(tune, monkeypatch)
-        eng.run(J.PHASE_EQUIL, 40)
+        eng.run(J.PHASE_EQUIL, 60)
         eng.run(J.PHASE_MEAS, 3)
         plan = eng.launch_plan()
         say(f"  plan: {plan}")

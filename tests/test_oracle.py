@@ -27,6 +27,44 @@ def test_philox_known_answers(ctr, key, out):
     assert tuple(int(v) for v in O.philox(ctr, key)) == out
 
 
+@pytest.mark.parametrize("L,H", [(8, 4), (16, 6), (6, 4), (10, 2)])
+def test_site_words_use_each_draw_once(L, H):
+    """The sweep's random words: every site of a colour gets exactly one
+    word; every 16-bit half of every call's four words goes to exactly one
+    site (L % 4 == 0; the short last group of a layer leaves some unused
+    otherwise), high parts from the calls whose second counter word is
+    c | rep << 2, low parts from the calls with bit 30 added.  Mistakes such
+    as two layers taking the same half, or the low part coming from the high
+    part's call, fail here."""
+    seed, iconf, rep, sweep = 12345, 7, 3, (5 << 32) + 9
+    half = L // 2
+    lay, ncol = half * half, L * L * H // 4
+    gl = (lay + 3) // 4
+    for c in range(4):
+        seen = np.zeros(ncol, np.int64)
+        his, los, drawn_hi, drawn_lo = [], [], [], []
+        for m in range(H // 2 * gl):
+            qi, r = O.call_words(L, seed, iconf, rep, sweep, c, m)
+            seen[qi] += 1
+            his += list(r >> 16)
+            los += list(r & 0xFFFF)
+            ctr = (m, c | rep << 2, sweep & 0xFFFFFFFF, sweep >> 32)
+            hw = O.philox(ctr, (seed, iconf))
+            lw = O.philox((ctr[0], ctr[1] | 0x40000000, ctr[2], ctr[3]), (seed, iconf))
+            # layer 2zp takes the high halves, layer 2zp + 1 the low halves
+            n_even = sum(1 for q in qi if (q // lay) % 2 == 0)
+            assert all((q // lay) // 2 == m // gl for q in qi)
+            assert list(r[:n_even] >> 16) == [int(w) >> 16 for w in hw[:n_even]]
+            assert list(r[n_even:] >> 16) == [int(w) & 0xFFFF for w in hw[:len(qi) - n_even]]
+            drawn_hi += [int(w) >> 16 for w in hw] + [int(w) & 0xFFFF for w in hw]
+            drawn_lo += [int(w) >> 16 for w in lw] + [int(w) & 0xFFFF for w in lw]
+        assert np.all(seen == 1)
+        if L % 4 == 0:
+            assert sorted(his) == sorted(drawn_hi) and sorted(los) == sorted(drawn_lo)
+        # the low parts are not the high parts' halves
+        assert sum(a == b for a, b in zip(los, his)) < 3
+
+
 def test_exp64_accuracy():
     xs = np.concatenate([np.linspace(-700, 0, 20001), np.linspace(-1, 1, 2001), [0.0, -1e-300]])
     for x in xs:
