@@ -60,6 +60,23 @@ __host__ __device__ inline int64_t internal_index(int32_t L, int32_t H, int32_t 
     return c * nq + ((int64_t)z * half + (y >> 1)) * half + (x >> 1);
 }
 
+// Spins in HBM of the bit-layout sweep's lattices (LatDesc::packed == 2): its
+// LDS unit words as they are.  Half-row unit u (internal sites u HW .. u HW +
+// HW - 1, HW = L/2) is two dwords at dwords 2u, 2u + 1: sites m < N0 at bits
+// 1..N0 of dword 0, sites N0..HW-1 at bits 1..HW-N0 of dword 1, where N0 =
+// 4 ceil(G/2) for G = HW/4 four-site groups; bit 0 of each dword repeats the
+// site before its first one and the bit after its last site the site after it
+// (dword 0: sites HW-1 and N0, dword 1: sites N0-1 and 0).  Dword and bit of
+// internal site i:
+__host__ __device__ inline void unit_word_bit(int32_t L, int64_t i, int64_t &dw, int &bit)
+{
+    const int HW = L >> 1, N0 = 4 * ((HW / 4 + 1) / 2);
+    const int64_t u = i / HW;
+    const int m = (int)(i - u * HW);
+    dw = 2 * u + (m >= N0 ? 1 : 0);
+    bit = (m >= N0 ? m - N0 : m) + 1;
+}
+
 // Inverse of internal_index.
 __host__ __device__ inline void site_of_internal(int32_t L, int32_t H, int64_t i, int &x, int &y, int &z)
 {

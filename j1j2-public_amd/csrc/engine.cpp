@@ -988,6 +988,56 @@ void from_internal(int L, int H, const S *in, D *ref, D (*conv)(S))
             for (int x = 0; x < L; ++x) ref[ref_site(L, x, y, z)] = conv(in[internal_index(L, H, x, y, z)]);
 }
 
+// Bytes of a lattice's spins in HBM: one per site, one bit per site (packed
+// 1), or the bit-layout sweep's two dwords per half-row unit (packed 2, see
+// unit_word_bit).
+int64_t spin_layout_bytes(const LatDesc &d)
+{
+    if (d.packed == 2) return 8 * (int64_t)d.N / (d.L / 2);
+    return d.packed ? d.N / 8 : d.N;
+}
+
+// Internal-order spin bytes (0/1) -> the lattice's packed form.  Unit words
+// also carry each dword's copies of the sites just outside it.
+void pack_spins(const LatDesc &d, const uint8_t *in, std::vector<uint8_t> &out)
+{
+    out.assign((size_t)spin_layout_bytes(d), 0);
+    if (d.packed == 1) {                              // site i at bit i & 7 of byte i >> 3
+        for (int64_t i = 0; i < d.N; ++i) out[i >> 3] |= (uint8_t)(in[i] << (i & 7));
+        return;
+    }
+    const int HW = d.L / 2, N0 = 4 * ((HW / 4 + 1) / 2), N1 = HW - N0;
+    const int64_t nu = (int64_t)d.N / HW;
+    std::vector<uint32_t> w((size_t)(2 * nu));
+    for (int64_t u = 0; u < nu; ++u) {
+        const uint8_t *s = in + u * HW;
+        uint32_t a = (uint32_t)s[HW - 1] | ((uint32_t)s[N0] << (N0 + 1));
+        uint32_t b = (uint32_t)s[N0 - 1] | ((uint32_t)s[0] << (N1 + 1));
+        for (int m = 0; m < N0; ++m) a |= (uint32_t)s[m] << (m + 1);
+        for (int m = N0; m < HW; ++m) b |= (uint32_t)s[m] << (m - N0 + 1);
+        w[2 * u] = a;
+        w[2 * u + 1] = b;
+    }
+    std::memcpy(out.data(), w.data(), out.size());
+}
+
+// The lattice's packed form -> internal-order spin bytes (0/1).
+void unpack_spins(const LatDesc &d, const uint8_t *in, uint8_t *out)
+{
+    if (d.packed == 1) {
+        for (int64_t i = 0; i < d.N; ++i) out[i] = (in[i >> 3] >> (i & 7)) & 1;
+        return;
+    }
+    for (int64_t i = 0; i < d.N; ++i) {
+        int64_t dw;
+        int bit;
+        unit_word_bit(d.L, i, dw, bit);
+        uint32_t v;
+        std::memcpy(&v, in + 4 * dw, 4);
+        out[i] = (uint8_t)((v >> bit) & 1u);
+    }
+}
+
 int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **handle)
 {
     if (!handle) return fail(J1J2_EINVAL, "null handle pointer");
@@ -1066,9 +1116,11 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
                 LatDesc d{};
                 d.L = p.L; d.H = p.H; d.N = G.N; d.conf = (int32_t)E->confs.size(); d.rep = r;
                 d.slot_base = cd.slot_base; d.spin_off = E->spin_bytes; d.site_off = cd.site_off;
-                d.packed = G.kind == 1 ? 1 : 0;       // the byte-layout resident sweep keeps one bit per site
+                // the byte-layout resident sweep keeps one bit per site, the
+                // bit-layout sweep its unit words
+                d.packed = G.kind == 1 ? 1 : (G.kind == 2 ? 2 : 0);
                 E->lats.push_back(d);
-                E->spin_bytes += ((int64_t)(d.packed ? G.N / 8 : G.N) + 255) / 256 * 256;
+                E->spin_bytes += (spin_layout_bytes(d) + 255) / 256 * 256;
                 ++nlat;
             }
             E->nslots += p.nreplica;
@@ -1332,17 +1384,16 @@ int j1j2_set_spins(void *handle, int32_t conf, const int8_t *spins, int32_t nrep
     for (int64_t i = 0; i < (int64_t)nrep * cd.N; ++i)
         if (spins[i] != 1 && spins[i] != -1) return fail(J1J2_EINVAL, "spins must be +1 or -1");
     HIPCHK(hipSetDevice(E->device));
-    std::vector<uint8_t> buf(cd.N), bits((size_t)cd.N / 8 + 1);
+    std::vector<uint8_t> buf(cd.N), packed;
     for (int r = 0; r < nrep; ++r) {
         const LatDesc &d = E->lats[cd.lat0 + r];
         to_internal(cd.L, cd.H, spins + (int64_t)r * cd.N, buf.data());
         const uint8_t *src = buf.data();
-        if (d.packed) {                                   // site i at bit i & 7 of byte i >> 3
-            std::fill(bits.begin(), bits.end(), 0);
-            for (int64_t i = 0; i < cd.N; ++i) bits[i >> 3] |= (uint8_t)(buf[i] << (i & 7));
-            src = bits.data();
+        if (d.packed) {
+            pack_spins(d, buf.data(), packed);
+            src = packed.data();
         }
-        HIPCHK(hipMemcpyAsync(E->st.spins + d.spin_off, src, d.packed ? cd.N / 8 : cd.N, hipMemcpyHostToDevice,
+        HIPCHK(hipMemcpyAsync(E->st.spins + d.spin_off, src, spin_layout_bytes(d), hipMemcpyHostToDevice,
                               E->stream));
         HIPCHK(hipStreamSynchronize(E->stream));
     }
@@ -1405,14 +1456,14 @@ int j1j2_get_spins(void *handle, int32_t conf, int8_t *spins, int32_t nrep)
     const ConfDesc &cd = E->confs[conf];
     if (!spins || nrep != cd.nrep) return fail(J1J2_EINVAL, "spins must hold NREPLICA*L*L*H values");
     HIPCHK(hipSetDevice(E->device));
-    std::vector<uint8_t> buf(cd.N), bits((size_t)cd.N / 8 + 1);
+    std::vector<uint8_t> buf(cd.N), packed;
     for (int r = 0; r < nrep; ++r) {
         const LatDesc &d = E->lats[cd.lat0 + r];
-        HIPCHK(hipMemcpyAsync(d.packed ? bits.data() : buf.data(), E->st.spins + d.spin_off,
-                              d.packed ? cd.N / 8 : cd.N, hipMemcpyDeviceToHost, E->stream));
+        packed.assign((size_t)spin_layout_bytes(d), 0);
+        HIPCHK(hipMemcpyAsync(d.packed ? packed.data() : buf.data(), E->st.spins + d.spin_off,
+                              spin_layout_bytes(d), hipMemcpyDeviceToHost, E->stream));
         HIPCHK(hipStreamSynchronize(E->stream));
-        if (d.packed)
-            for (int64_t i = 0; i < cd.N; ++i) buf[i] = (bits[i >> 3] >> (i & 7)) & 1;
+        if (d.packed) unpack_spins(d, packed.data(), buf.data());
         from_internal<uint8_t, int8_t>(cd.L, cd.H, buf.data(), spins + (int64_t)r * cd.N,
                                        [](uint8_t b) -> int8_t { return b ? 1 : -1; });
     }

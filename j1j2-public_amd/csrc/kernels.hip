@@ -59,10 +59,12 @@ __device__ __forceinline__ UnitGeo unit_geo(int u, int c, int H)
 }
 
 // Spins in HBM: one byte b = (s+1)/2 per site in the internal (colour-plane)
-// order, or, for the lattices of the byte-layout resident sweep, one bit per
-// site (bit i & 31 of word i >> 5 for internal index i), which that sweep
-// loads and stores once per cycle.  Dword idx of the byte form (sites
-// 4 idx .. 4 idx + 3) from either: PK = 1 reads nibble idx of the bits.
+// order; for the lattices of the byte-layout resident sweep (packed = 1) one
+// bit per site (bit i & 31 of word i >> 5 for internal index i), which that
+// sweep loads and stores once per cycle; for those of the bit-layout sweep
+// (packed = 2) its unit words (unit_word_bit).  Dword idx of the byte form
+// (sites 4 idx .. 4 idx + 3) from the first two: PK = 1 reads nibble idx of
+// the bits.
 template <int PK>
 __device__ __forceinline__ uint32_t rd_word(const uint32_t *sp32, int idx)
 {
@@ -70,9 +72,16 @@ __device__ __forceinline__ uint32_t rd_word(const uint32_t *sp32, int idx)
     else return sp32[idx];
 }
 
-__device__ __forceinline__ uint32_t spin_at(const uint8_t *sp, int packed, int64_t i)
+__device__ __forceinline__ uint32_t spin_at(const uint8_t *sp, int packed, int L, int64_t i)
 {
-    return packed ? (reinterpret_cast<const uint32_t *>(sp)[i >> 5] >> (i & 31)) & 1u : sp[i];
+    const uint32_t *sp32 = reinterpret_cast<const uint32_t *>(sp);
+    if (packed == 2) {
+        int64_t dw;
+        int bit;
+        unit_word_bit(L, i, dw, bit);
+        return (sp32[dw] >> bit) & 1u;
+    }
+    return packed ? (sp32[i >> 5] >> (i & 31)) & 1u : sp[i];
 }
 
 // +x / -x neighbour bytes of dword w of the half-row at dword offset `uoff`,
@@ -931,10 +940,10 @@ __global__ __launch_bounds__(OBS_THREADS) void k_observables(DeviceState st, int
         int x, y, z;
         site_of_internal(L, H, i, x, y, z);
         const Nbr nb = neighbours(L, H, x, y, z);
-        const int si = spin_at(sp, d.packed, i) ? 1 : -1;
+        const int si = spin_at(sp, d.packed, L, i) ? 1 : -1;
         int s[10];
 #pragma unroll
-        for (int t = 0; t < 10; ++t) s[t] = spin_at(sp, d.packed, nb.m[t]) ? 1 : -1;
+        for (int t = 0; t < 10; ++t) s[t] = spin_at(sp, d.packed, L, nb.m[t]) ? 1 : -1;
         v[OBS_M] += si;
         v[OBS_SX] += (x & 1) ? -si : si;
         v[OBS_SY] += (y & 1) ? -si : si;
@@ -1141,7 +1150,7 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
     int32_t *ls = st.locspin + d.site_off, *ln = st.locnem + d.site_off;
     const int t0 = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
     const int pk = d.packed;
-    switch (pk ? L : L + 1) {     // packed lattices: L % 8 == 0, L <= 48
+    switch (pk == 1 ? L : (pk == 0 ? L + 1 : 0)) {   // bits: L % 8 == 0, L <= 48; unit words: the loop
         case 8: locnem_units<1, 0, 1>(sp32, H, t0, stride, ls, ln); break;
         case 16: locnem_units<2, 0, 1>(sp32, H, t0, stride, ls, ln); break;
         case 24: locnem_units<3, 0, 1>(sp32, H, t0, stride, ls, ln); break;
@@ -1159,9 +1168,9 @@ __global__ __launch_bounds__(256) void k_locnem(DeviceState st, int32_t only_pen
                 int x, y, z;
                 site_of_internal(L, H, i, x, y, z);
                 const Nbr nb = neighbours(L, H, x, y, z);
-                const int si = spin_at(sp, pk, i) ? 1 : -1;
-                const int l = si * ((spin_at(sp, pk, nb.m[2]) ? 1 : -1) - (spin_at(sp, pk, nb.m[0]) ? 1 : -1) +
-                                    (spin_at(sp, pk, nb.m[3]) ? 1 : -1) - (spin_at(sp, pk, nb.m[1]) ? 1 : -1));
+                const int si = spin_at(sp, pk, L, i) ? 1 : -1;
+                const int l = si * ((spin_at(sp, pk, L, nb.m[2]) ? 1 : -1) - (spin_at(sp, pk, L, nb.m[0]) ? 1 : -1) +
+                                    (spin_at(sp, pk, L, nb.m[3]) ? 1 : -1) - (spin_at(sp, pk, L, nb.m[1]) ? 1 : -1));
                 ls[i] += si;
                 ln[i] += l;
             }

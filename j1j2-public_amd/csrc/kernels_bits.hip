@@ -20,7 +20,9 @@ namespace {
 // first one (dword 0: site HW-1, dword 1: site N0-1) and bit N+1 the site
 // after its last one (dword 0: site N0, dword 1: site 0).  So the four sites
 // of a group and the same window shifted by one site either way (its x
-// neighbours, wrap included) are one 4-bit field of one dword.
+// neighbours, wrap included) are one 4-bit field of one dword.  The lattice's
+// spins in HBM are the same unit words (unit_word_bit), so loading and storing
+// a lattice is a copy of 8 bytes per half-row.
 //
 // A lattice may instead be split into S z-slabs, one block each (all of them
 // resident at once): a slab holds its own layers plus one halo layer below and
@@ -444,17 +446,16 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
                                           uint32_t *lds32, int64_t (*red)[OBS_N])
 {
     using Gm = BitGeo<L>;
-    constexpr int HW = Gm::HW, G = Gm::G, NT = Gm::NT;
+    constexpr int HW = Gm::HW, NT = Gm::NT;
     const int H = d.H, tid = threadIdx.x;
     const int NUg = HW * H;
     Share sh;
     sh.nz = SLAB ? H / S : H;
     sh.z0 = SLAB ? s * sh.nz : 0;
     sh.NU = SLAB ? HW * (sh.nz + 2) : NUg;
-    uint8_t *gsp = st.spins + d.spin_off;
-
-    // bytes (colour-plane order, global unit c*NUg + z*HW + yh at byte HW times
-    // that) -> unit words; a slab takes its layers and the two halo layers
+    // the lattice in HBM is its unit words (global unit c*NUg + z*HW + yh);
+    // a slab takes its layers and the two halo layers
+    uint2 *gu = reinterpret_cast<uint2 *>(st.spins + d.spin_off);
     for (int U = tid; U < 4 * sh.NU; U += NT) {
         const int c = U / sh.NU, r = U - c * sh.NU, li = r / HW, yh = r - li * HW;
         int zg = li;
@@ -462,14 +463,7 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
             zg = sh.z0 - 1 + li;
             zg = zg < 0 ? zg + H : (zg >= H ? zg - H : zg);
         }
-        const uint8_t *src = gsp + ((int64_t)c * NUg + (int64_t)zg * HW + yh) * HW;
-        unsigned long long v = 0;
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const uint32_t b = *reinterpret_cast<const uint32_t *>(src + 4 * k);
-            v |= (unsigned long long)((b * 0x10204080u) >> 28) << (4 * k);   // bytes 0..3 -> bits
-        }
-        reinterpret_cast<uint2 *>(lds32)[U] = unit_from_sites<L>(v);
+        reinterpret_cast<uint2 *>(lds32)[U] = gu[(int64_t)c * NUg + (int64_t)zg * HW + yh];
     }
     if (tid == 0) lds32[8 * sh.NU] = 0;                         // the spare dword
 
@@ -571,17 +565,11 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
             atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
         }
     }
-    // unit words -> bytes (a slab stores its own layers)
+    // unit words back to HBM (a slab stores its own layers)
     const int off = SLAB ? HW : 0;
     for (int U = tid; U < 4 * sh.nz * HW; U += NT) {
         const int c = U / (sh.nz * HW), r = U - c * sh.nz * HW;
-        const unsigned long long v = sites_from_unit<L>(reinterpret_cast<const uint2 *>(lds32)[c * sh.NU + off + r]);
-        uint8_t *dst = gsp + ((int64_t)c * NUg + (int64_t)sh.z0 * HW + r) * HW;
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const uint32_t n4 = (uint32_t)(v >> (4 * k)) & 0xFu;
-            *reinterpret_cast<uint32_t *>(dst + 4 * k) = (n4 * 0x00204081u) & 0x01010101u;
-        }
+        gu[(int64_t)c * NUg + (int64_t)sh.z0 * HW + r] = reinterpret_cast<const uint2 *>(lds32)[c * sh.NU + off + r];
     }
     block_reduce_obs<NT>(v, red);
     if (tid == 0) {
