@@ -152,6 +152,66 @@ __device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int 
     return s;
 }
 
+// The operands of the field sums of one 4-site group (fast path): the own
+// nibble, the f16 views of the x and y neighbours (site bit at bit 10 of its
+// half, as f16_even / f16_odd of the byte form) and of the bond counts Z and
+// D.  The x neighbours and the diagonals come from 5-site windows of one dword
+// starting at the lower x (bit k of a window is the -x side of site k, bit
+// k + 1 its +x side), so each is one extract: a multiply by 0x00810204 puts
+// window bit k at bit 8k+2 and bit k+1 at bit 8k+3; a multiply by 0x108421
+// puts window bit b at bit 6b (no two products overlap, so no carries), the
+// two windows' sum u has the per-bit counts 0..2 in 6-bit fields and u + (u <<
+// 6) the diagonal count D_k = u_k + u_{k+1} in field k+1.  Views of D: D_0
+// and D_1 at bit 6 of the low half (f16 value D 2^-18), D_2 and D_3 at bit 2
+// of the high half (D 2^-22), against the coefficients 2^18 (2 J2) and 2^22
+// (2 J2): the products are exactly D 2 J2, as in the byte form (4 D per byte
+// against 2^22 (2 J2) / 4), so every fma rounds as before.
+struct BitView {
+    uint32_t S, xpe, xpo, xme, xmo, ype, ypo, yme, ymo, ze, zod, de, dod;
+};
+
+__device__ __forceinline__ uint32_t spread6(uint32_t w5) { return (w5 * 0x108421u) & 0x01041041u; }
+
+template <int HW, int HX>
+__device__ __forceinline__ BitView bit_view(const uint32_t *lds32, int c, int NU, const BitNb &n, GroupPos gp)
+{
+    const int pos0 = gp.pos, pmin = HX ? gp.pos : gp.pos - 1;
+    const int p1 = 2 * (c ^ 1) * NU + gp.h, p2 = 2 * (c ^ 2) * NU + gp.h, p3 = 2 * (c ^ 3) * NU + gp.h;
+    BitView v;
+    v.S = nib(lds32, 2 * (c * NU + n.u) + gp.h, pos0);
+    const uint32_t tx = __builtin_amdgcn_ubfe(lds32[p1 + 2 * n.u], (uint32_t)pmin, 5u) * 0x00810204u;
+    const uint32_t txp = tx >> 1;
+    v.xme = f16_even(tx & 0x04040404u);
+    v.xmo = tx & 0x04000400u;
+    v.xpe = f16_even(txp & 0x04040404u);
+    v.xpo = txp & 0x04000400u;
+    const uint32_t yp = spread4(nib(lds32, p2 + 2 * n.uyp, pos0)), ym = spread4(nib(lds32, p2 + 2 * n.uym, pos0));
+    v.ype = f16_even(yp);
+    v.ypo = f16_odd(yp);
+    v.yme = f16_even(ym);
+    v.ymo = f16_odd(ym);
+    const uint32_t Z = spread4(nib(lds32, p2 + 2 * n.uzp, pos0)) + spread4(nib(lds32, p2 + 2 * n.uzm, pos0));
+    v.ze = lo_even(Z);
+    v.zod = lo_odd(Z);
+    const uint32_t u = spread6(__builtin_amdgcn_ubfe(lds32[p3 + 2 * n.uyp], (uint32_t)pmin, 5u)) +
+                       spread6(__builtin_amdgcn_ubfe(lds32[p3 + 2 * n.uym], (uint32_t)pmin, 5u));
+    const uint32_t Dw = u + (u << 6);
+    v.de = Dw & 0x001C01C0u;
+    v.dod = (Dw >> 6) & 0x001C01C0u;
+    return v;
+}
+
+// The same operands from the spread neighbours (FP64 shadow path: D in the
+// byte form, coefficient 2^22 (2 J2) for every site).
+__device__ __forceinline__ BitView view_of_sites(const BitSites &s)
+{
+    const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
+    const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
+    return BitView{s.S,           f16_even(s.xp), f16_odd(s.xp), f16_even(s.xm), f16_odd(s.xm),
+                   f16_even(s.yp), f16_odd(s.yp), f16_even(s.ym), f16_odd(s.ym), lo_even(Z),
+                   lo_odd(Z),      lo_even(D),    lo_odd(D)};
+}
+
 __device__ __forceinline__ uint32_t nib_bit(uint32_t n, int k) { return (n >> k) & 1u; }
 __device__ __forceinline__ uint32_t sp_bit(uint32_t w, int k) { return (w >> (8 * k + 2)) & 1u; }
 
@@ -168,10 +228,19 @@ struct Share {
 // of Philox call m = (zg / 2) TPL + tid (low halves from the call with LO_CTR
 // when needed).  Philox counters and disorder planes are indexed by the global
 // layer zg, so a slab updates its sites exactly as the whole lattice would.
+struct KPlanes {
+    float4 kc, kxp, kxm, kyp, kym;
+};
+
+__device__ __forceinline__ KPlanes load_kplanes(const SweepCtx &cx, uint32_t ob)
+{
+    return KPlanes{ld16(cx.kc4, ob), ld16(cx.kxp4, ob), ld16(cx.kxm4, ob), ld16(cx.kyp4, ob), ld16(cx.kym4, ob)};
+}
+
 template <int L, int HX, bool SHADOW, bool SLAB, int ODD>
 __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int c, int H,
                                           int NU, int zg, int zl, int yh, int j, uint32_t m, const U4 &hi,
-                                          unsigned long long &sh_prop, unsigned long long &sh_mis)
+                                          const KPlanes &kp, unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
     using Gm = BitGeo<L>;
     constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
@@ -179,17 +248,19 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     const int NUg = HW * H, nq = NUg * G;
     const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
     const GroupPos gp = group_pos<L>(j);
-    const BitSites s = bit_sites<HW, HX>(lds32, c, NU, n, gp);
-    const uint32_t D = (s.d1p + s.d1m) + (s.d2p + s.d2m);     // bytewise 4 * (0..4)
-    const uint32_t Z = s.zp + s.zm;                           // bytewise 4 * (0..2)
-    const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
-    const uint32_t xpe = f16_even(s.xp), xpo = f16_odd(s.xp), xme = f16_even(s.xm), xmo = f16_odd(s.xm);
-    const uint32_t ype = f16_even(s.yp), ypo = f16_odd(s.yp), yme = f16_even(s.ym), ymo = f16_odd(s.ym);
+    BitSites s{};                                             // spread neighbours: FP64 shadow only
+    BitView v;
+    if constexpr (SHADOW) {
+        s = bit_sites<HW, HX>(lds32, c, NU, n, gp);
+        v = view_of_sites(s);
+    } else {
+        v = bit_view<HW, HX>(lds32, c, NU, n, gp);
+    }
+    // D coefficients per site (the fast path's D_0, D_1 sit 2^4 higher)
+    const float dlo = SHADOW ? cx.j2d : cx.j2d * 0x1p-4f;
+    const float dk[4] = {dlo, dlo, cx.j2d, cx.j2d};
     const int own = c * nq + zg * TPL + tid;                  // the group's dword in the colour-plane order
-    uint32_t ob = (uint32_t)own * 16u;                        // formed here, not hoisted out of the sweep loop
-    asm volatile("" : "+v"(ob));
-    const float4 kc = ld16(cx.kc4, ob), kxp = ld16(cx.kxp4, ob), kxm = ld16(cx.kxm4, ob),
-                 kyp = ld16(cx.kyp4, ob), kym = ld16(cx.kym4, ob);
+    const float4 kc = kp.kc, kxp = kp.kxp, kxm = kp.kxm, kyp = kp.kyp, kym = kp.kym;
     const float kca[4] = {kc.x, kc.y, kc.z, kc.w};
     const float kxpa[4] = {kxp.x, kxp.y, kxp.z, kxp.w};
     const float kxma[4] = {kxm.x, kxm.y, kxm.z, kxm.w};
@@ -207,20 +278,20 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     // the four sites' fields term by term (each site's sum in the same order):
     // independent neighbours in the issue stream, no dependent fma pairs
 #pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(xpe, xpo, k), kxpa[k], kca[k]);
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(v.xpe, v.xpo, k), kxpa[k], kca[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(xme, xmo, k), kxma[k], h[k]);
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(v.xme, v.xmo, k), kxma[k], h[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(ype, ypo, k), kypa[k], h[k]);
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(v.ype, v.ypo, k), kypa[k], h[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(yme, ymo, k), kyma[k], h[k]);
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(v.yme, v.ymo, k), kyma[k], h[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(de, dod, k), cx.j2d, h[k]);
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(v.de, v.dod, k), dk[k], h[k]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(ze, zod, k), cx.j3d, h[k]);
+    for (int k = 0; k < 4; ++k) h[k] = __builtin_fmaf(f16_site(v.ze, v.zod, k), cx.j3d, h[k]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        ys[k] = y_f32_word(h[k], s.S, k, cx.b2);
+        ys[k] = y_f32_word(h[k], v.S, k, cx.b2);
         if constexpr (!SHADOW) {
             const Filt fl = decide16_filter(ys[k], v16_float(ha[k], ODD));
             f |= (fl.acc ? 1u : 0u) << k;
@@ -277,16 +348,25 @@ __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, co
                                          const Share &sh, unsigned long long &sh_prop, unsigned long long &sh_mis)
 {
     using Gm = BitGeo<L>;
-    constexpr int G = Gm::G, TPL = Gm::TPL;
+    constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G;
+    const int nq = HW * H * G;
     for (int zl = 0; zl < sh.nz; zl += 2) {
         const int zg = sh.z0 + zl;
         const uint32_t m = (uint32_t)((zg >> 1) * TPL + tid);
+        // the first group's planes are requested before its Philox call, the
+        // second group's after the first group's fields (their latency hides
+        // behind that work)
+        uint32_t ob = (uint32_t)(c * nq + zg * TPL + tid) * 16u;   // formed here, not hoisted out of the loop
+        asm volatile("" : "+v"(ob));
+        const KPlanes k0 = load_kplanes(cx, ob);
         const U4 hi = philox4x32_10_u(m, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
-        bit_group<L, HX, SHADOW, SLAB, 0>(lds32, cx, cc, c, H, sh.NU, zg, zl, yh, j, m, hi, sh_prop, sh_mis);
-        bit_group<L, HX, SHADOW, SLAB, 1>(lds32, cx, cc, c, H, sh.NU, zg + 1, zl + 1, yh, j, m, hi, sh_prop, sh_mis);
+        bit_group<L, HX, SHADOW, SLAB, 0>(lds32, cx, cc, c, H, sh.NU, zg, zl, yh, j, m, hi, k0, sh_prop, sh_mis);
+        const KPlanes k1 = load_kplanes(cx, ob + 16u * TPL);
+        bit_group<L, HX, SHADOW, SLAB, 1>(lds32, cx, cc, c, H, sh.NU, zg + 1, zl + 1, yh, j, m, hi, k1, sh_prop,
+                                          sh_mis);
     }
 }
 
