@@ -101,17 +101,16 @@ def mismatch_rate(J, cycles):
 PMC_SUMMARY = os.path.join("profiles", "k_sweep_resident_l32_16x64_pmc.md")
 
 
-def traffic_from_profiles():
-    """HBM bytes per sweep-kernel launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    and the block size of the kernel variant those counters were read on."""
+def traffic_from_profiles(nt):
+    """HBM bytes per measuring sweep launch of the nt-thread variant from the
+    committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when the summary has no figure for
+    that block size."""
     path = os.path.join(REPO, PMC_SUMMARY)
-    if not os.path.exists(path):
-        return None, None
-    text = open(path).read()
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", text, re.M)
-    nt = re.search(r"^block_threads:\s*(\d+)", text, re.M)
-    return (float(m.group(1)) if m else None), (int(nt.group(1)) if nt else None)
+    if nt is None or not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\[%d\]:\s*([0-9.eE+]+)" % nt, open(path).read(), re.M)
+    return float(m.group(1)) if m else None
 
 
 def cpu_model():
@@ -387,8 +386,9 @@ def main():
     su_launch = fp32["su"] / max(fp32["launches"], 1)
     avg_s = fp32["sweep_ms"] / 1e3 / max(fp32["launches"], 1)
     achieved = su_launch * fp32["bytes_per"] / avg_s
-    traffic, traffic_nt = traffic_from_profiles()
-    timed_nt = re.search(r"nt(\d+)", fp32["plan"])
+    m_nt = re.search(r"nt(\d+)", fp32["plan"])
+    timed_nt = int(m_nt.group(1)) if m_nt else None
+    traffic = traffic_from_profiles(timed_nt)
     res["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK, "traffic": traffic,
                        "kernel": "k_sweep_resident", "avg_launch_ms": avg_s * 1e3,
@@ -399,8 +399,8 @@ def main():
                        # counters' traffic, read on the variant named here
                        "achieved_basis": "algorithmic bytes (6 B per site update) / average launch time",
                        "traffic_source": PMC_SUMMARY,
-                       "traffic_block_threads": traffic_nt,
-                       "timed_block_threads": int(timed_nt.group(1)) if timed_nt else None,
+                       "traffic_block_threads": timed_nt if traffic else None,
+                       "timed_block_threads": timed_nt,
                        "measured_hbm_GBps": (traffic / avg_s / 1e9) if traffic else None,
                        "limiter": "VALU issue (the lattice stays in LDS for the launch's 5 sweeps)"}
     if not args.no_fp64:
