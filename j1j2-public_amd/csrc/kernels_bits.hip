@@ -372,41 +372,57 @@ __device__ __forceinline__ void bit_pass(uint32_t *lds32, const SweepCtx &cx, co
 
 // Per-thread partial observables of the bit layout (same sums as ObsFast in
 // kernels.hip, one 4-site group per step; +z bonds, so a slab reads its upper
-// halo).
-template <int L, int HX, bool SLAB>
-__device__ __forceinline__ void bit_observe(const uint32_t *lds32, int c, int H, const Share &sh,
-                                            const QPlanes &qp, ObsFast &a)
+// halo), for colour C.  The bond counts are popcounts of nibble xors (the x
+// and diagonal neighbours from 5-site windows as in the sweep); the field
+// term's e_k = a(+y) + a(-y) - a(+x) - a(-x) + 2 is formed in bytes from the
+// four anti-aligned nibbles.  Layers go in pairs: the first of a pair has y
+// parity C >> 1 (a block's first layer is even), the second the other one.
+__device__ __forceinline__ uint32_t spread_b0(uint32_t n) { return (n * 0x00204081u) & 0x01010101u; }
+
+template <int L, int C, bool SLAB>
+__device__ __forceinline__ void bit_observe(const uint32_t *lds32, int H, const Share &sh, const QPlanes &qp,
+                                            ObsFast &a)
 {
     using Gm = BitGeo<L>;
-    constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL;
+    constexpr int HW = Gm::HW, G = Gm::G, TPL = Gm::TPL, HX = C & 1, P0 = C >> 1;
     const int tid = threadIdx.x;
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G;
-    const int nq = HW * H * G;
-    uint32_t ps = 0, px = 0, py = 0, pz = 0, pd = 0, nw = 0;
-    int32_t sxm = 0, sym = 0;
-    for (int zl = 0; zl < sh.nz; ++zl) {
+    const int nq = HW * H * G, NU = sh.NU;
+    const GroupPos gp = group_pos<L>(j);
+    const uint32_t pos0 = (uint32_t)gp.pos, pmin = HX ? pos0 : pos0 - 1u;
+    const int q0 = 2 * C * NU + gp.h, q1 = 2 * (C ^ 1) * NU + gp.h, q2 = 2 * (C ^ 2) * NU + gp.h,
+              q3 = 2 * (C ^ 3) * NU + gp.h;
+    uint32_t ps0 = 0, ps1 = 0, px = 0, py = 0, pz = 0, pd = 0;
+    const auto layer = [&](int zl, int par, uint32_t &ps) {
         const int zg = sh.z0 + zl;
-        const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, group_pos<L>(j));
-        const uint32_t S = spread4(s.S);
-        const uint32_t sx0 = S ^ s.xp, sy0 = S ^ s.yp;
-        const int pcs = __builtin_popcount(s.S);
-        ps += pcs;
-        px += __builtin_popcount(sx0);
-        py += __builtin_popcount(sy0);
-        pz += __builtin_popcount(S ^ s.zp);
-        pd += __builtin_popcount(S ^ s.d1p) + __builtin_popcount(S ^ s.d1m);
-        nw += 1;
-        const int m = 2 * pcs - 4;
-        sxm += HX ? -m : m;
-        sym += n.p ? -m : m;
-        const uint32_t ax = sx0 + (S ^ s.xm), ay = sy0 + (S ^ s.ym);   // bytes 0, 4, 8
-        const uint32_t e = ((ay + 0x08080808u) - ax) >> 2;             // bytes l/2 + 2
-        qsum_word(a.q, qp, (uint32_t)(c * nq + zg * TPL + tid) * 4u, e);
+        const int u = (SLAB ? zl + 1 : zl) * HW + yh;
+        const int yp = (P0 ^ par) ? (yh == HW - 1 ? u - (HW - 1) : u + 1) : u;
+        const int ym = (P0 ^ par) ? u : (yh == 0 ? u + (HW - 1) : u - 1);
+        const int zp = SLAB ? u + HW : (zl == H - 1 ? u - (H - 1) * HW : u + HW);
+        const uint32_t S = __builtin_amdgcn_ubfe(lds32[q0 + 2 * u], pos0, 4u);
+        const uint32_t wx = __builtin_amdgcn_ubfe(lds32[q1 + 2 * u], pmin, 5u);
+        const uint32_t wd = __builtin_amdgcn_ubfe(lds32[q3 + 2 * yp], pmin, 5u);
+        const uint32_t A = S ^ __builtin_amdgcn_ubfe(lds32[q2 + 2 * yp], pos0, 4u);   // +y anti-aligned
+        const uint32_t B = S ^ __builtin_amdgcn_ubfe(lds32[q2 + 2 * ym], pos0, 4u);   // -y
+        const uint32_t X = S ^ (wx >> 1), Y = S ^ (wx & 15u);                         // +x, -x
+        ps += __builtin_popcount(S);
+        px += __builtin_popcount(X);
+        py += __builtin_popcount(A);
+        pz += __builtin_popcount(S ^ __builtin_amdgcn_ubfe(lds32[q2 + 2 * zp], pos0, 4u));
+        pd += __builtin_popcount(S ^ (wd >> 1)) + __builtin_popcount(S ^ (wd & 15u));
+        const uint32_t e = (spread_b0(A) + spread_b0(B) + 0x02020202u) - (spread_b0(X) + spread_b0(Y));
+        qsum_word(a.q, qp, (uint32_t)(C * nq + zg * TPL + tid) * 4u, e);
+    };
+    for (int zl = 0; zl < sh.nz; zl += 2) {
+        layer(zl, 0, ps0);
+        layer(zl + 1, 1, ps1);
     }
-    a.ps += ps; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += nw;
-    a.sxm += sxm; a.sym += sym;
+    const int32_t half = sh.nz / 2;
+    const int32_t m0 = 2 * (int32_t)ps0 - 4 * half, m1 = 2 * (int32_t)ps1 - 4 * half;
+    a.ps += ps0 + ps1; a.px += px; a.py += py; a.pz += pz; a.pd += pd; a.nw += (uint32_t)sh.nz;
+    a.sxm += HX ? -(m0 + m1) : m0 + m1;
+    a.sym += P0 ? m1 - m0 : m0 - m1;
 }
 
 // Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) from the bit layout
@@ -623,10 +639,10 @@ __device__ __forceinline__ void bits_body(const DeviceState &st, const LatDesc &
 
     ObsFast a = {};
     const QPlanes qp = qplanes(st.qb + d.site_off, st.nsites);
-    for (int c = 0; c < 4; ++c) {
-        if (c & 1) bit_observe<L, 1, SLAB>(lds32, c, H, sh, qp, a);
-        else bit_observe<L, 0, SLAB>(lds32, c, H, sh, qp, a);
-    }
+    bit_observe<L, 0, SLAB>(lds32, H, sh, qp, a);
+    bit_observe<L, 1, SLAB>(lds32, H, sh, qp, a);
+    bit_observe<L, 2, SLAB>(lds32, H, sh, qp, a);
+    bit_observe<L, 3, SLAB>(lds32, H, sh, qp, a);
     const int64_t nw = (int64_t)a.nw;
     int64_t v[OBS_N];
     v[OBS_M] = 2 * (int64_t)a.ps - 4 * nw;
