@@ -19,7 +19,7 @@ struct LatDesc {          // one replica lattice
     int32_t conf;         // global config index
     int32_t rep;          // replica index within its config
     int32_t slot_base;    // index of the config's slot 0 in per-slot arrays
-    int32_t packed;       // spins in HBM: 0 one byte per site, 1 one bit per site
+    int32_t packed;       // spins in HBM: 0 one byte per site, 1 one bit per site, 2 unit words
     int64_t spin_off;     // byte offset of this lattice's spins
     int64_t site_off;     // offset (in sites) of the config's per-site planes
 };
@@ -249,19 +249,24 @@ __host__ __device__ inline uint32_t site_word(uint32_t hi, uint32_t lo, int odd)
 // v as the float 2^16 + v 2^-7 (v fills the low 16 bits of the mantissa).
 __device__ __forceinline__ float v16_float(uint32_t w, int odd) { return __uint_as_float(half16(w, odd) | 0x47800000u); }
 
-// The FP32 rule's decision from v alone, for every l: with F = 2^16 + v 2^-7
-// on the float grid of spacing 2^-7, F < RN(2^16 - 2^-7 + e 2^9 (1 - 2^-16))
-// gives (v + 1) 2^16 < 2^32 (1 - 2^-16) e, so every (float)r <= (v + 1) 2^16
-// is accepted (the bound of decide_f32_filter); F > RN(2^16 + e 2^9 (1 +
-// 2^-16)) gives v 2^16 > 2^32 (1 + 2^-16) e, so every (float)r >= v 2^16 is
-// rejected (below the normal range: v >= 1, r >= 2^16 > the threshold).  Open
-// otherwise (v within about 1 + 2e of 2^16 2^y: a fraction ~2^-16 of the
+// The FP32 rule's decision from v alone, for every l.  With F = 2^16 + v 2^-7
+// (exact) and e = v_exp_f32(y), d = RN(F - 2^9 e) is one fma, rounded once
+// (error <= 2^-9 below 2^16, <= 2^-8 above).  The rule accepts every l when
+// (v + 1) 2^16 <= T(y) (every (float)r is then <= (v + 1) 2^16, a float), and
+// T(y) and 2^32 e agree to 2^-22 relative, so for e <= 1 it suffices that
+// t = F - 2^9 e <= 2^16 - 2^-7 - 2^-13; d < 2^16 - 2^-7 means d <= 2^16 -
+// 3 2^-8 on the float grid, so t <= 2^16 - 5 2^-9 (for y > 0 the rule accepts
+// every move anyway).  It rejects every l when v 2^16 > T(y), for which t >
+// 2^16 + 2^-13 suffices; d > 2^16 means d >= 2^16 + 2^-7, so t >= 2^16 + 2^-8
+// (below the normal range: v >= 1, r >= 2^16 > the threshold).  Open
+// otherwise (v within about one of 2^16 2^y: a fraction ~2^-16 of the
 // proposals), decided by the exact rule on the whole word.
 __device__ __forceinline__ Filt decide16_filter(float y, float F)
 {
     const float e = __builtin_amdgcn_exp2f(y);
-    const bool acc = F < __builtin_fmaf(e, 0x1.fffep8f, 0x1.fffffcp15f);
-    const bool rej = F > __builtin_fmaf(e, 0x1.0001p9f, 0x1p16f);
+    const float d = __builtin_fmaf(e, -0x1p9f, F);
+    const bool acc = d < 0x1.fffffcp15f;             // 2^16 - 2^-7
+    const bool rej = d > 0x1p16f;                    // 2^16
     return Filt{acc, !(acc || rej)};
 }
 
