@@ -12,7 +12,10 @@ form's rate on the same workload, the sweep kernel's roofline figures and the
 reference's CPU rate on a bounded sample of the same workload.
 
 Before the timed region: the autotuner's cycles, --warmup equilibration
-cycles and 17 measuring cycles that build both measuring hipGraphs, so the
+cycles, further equilibration cycles until the GPU has been busy for
+CLOCK_WARM_S (its engine clock ramps up over the first ~0.1 s of sustained
+load: the sweep kernel runs ~10% slower before), and 31 measuring cycles
+that build all five measuring hipGraphs (16, 8, 4, 2 and 1 cycles), so the
 timed steps are steady-state replays.
 """
 import argparse
@@ -31,7 +34,8 @@ sys.path.insert(0, os.path.join(REPO, "j1j2-public_amd"))
 L, NREP, NCONF, PTSWAPEVER = 32, 16, 64, 5
 HBM_PEAK = 8.0e12                  # MI355X HBM3E, bytes/s
 TUNE_CYCLES = 60                   # the autotuner times its candidates on at most 54 cycles of one call
-GRAPH_WARM = 17                    # one 16-cycle graph launch and one single-cycle graph launch
+GRAPH_WARM = 31                    # one launch each of the 16-, 8-, 4-, 2- and 1-cycle graphs
+CLOCK_WARM_S = 0.5                 # untimed equilibration until the GPU has run this long
 FP64_MIN_STEPS = 20
 SEED = 20261203
 
@@ -52,10 +56,16 @@ def build_batch(J, precision, flags=0, nconf=NCONF):
 
 def warm(J, eng, warmup):
     """Untimed lead-in: the autotuner's cycles, `warmup` equilibration cycles,
-    then GRAPH_WARM measuring cycles, which capture and instantiate both
-    measuring hipGraphs (16 cycles and one cycle) before the timed region."""
+    more equilibration cycles until the GPU has been busy for CLOCK_WARM_S
+    (steady engine clock), then GRAPH_WARM measuring cycles, which capture
+    and instantiate the measuring hipGraphs (16, 8, 4, 2 and 1 cycles: a
+    call's cycles replay as 16-cycle graphs and one graph of each smaller
+    size for the remainder) before the timed region."""
+    t0 = time.perf_counter()
     eng.run(J.PHASE_EQUIL, TUNE_CYCLES)
     eng.run(J.PHASE_EQUIL, warmup)
+    while time.perf_counter() - t0 < CLOCK_WARM_S:
+        eng.run(J.PHASE_EQUIL, 32)                 # blocking
     eng.run_cycles(J.PHASE_MEAS, GRAPH_WARM, begin=True)
     eng.synchronize()
 

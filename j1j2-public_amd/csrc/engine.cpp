@@ -120,8 +120,8 @@ struct Engine {
     bool observed = false;                     // observables valid for the current state
     std::vector<void *> allocs;
     DeviceState st{};
-    // replayed schedules: graph[measure][0] = one cycle, [measure][1] = GRAPH_CYCLES cycles
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // replayed schedules: graph[measure][k] = 2^k cycles (k < GRAPH_SIZES)
+    hipGraphExec_t graph[2][5] = {};
     Timer timers[4];
 
     ~Engine()
@@ -469,14 +469,18 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
     return 0;
 }
 
-constexpr int GRAPH_CYCLES = 16;
+// A call's cycles replay as graphs of 16 cycles and then one graph each of
+// 8, 4, 2, 1 cycles for the remainder (at most 4 more launches: each graph
+// launch costs tens of microseconds of GPU-side gap, which single-cycle
+// graphs would pay per cycle).
+constexpr int GRAPH_SIZES = 5;
 
-int build_graph(Engine &E, int measure, int big)
+int build_graph(Engine &E, int measure, int lg)
 {
-    if (E.graph[measure][big]) return 0;
+    if (E.graph[measure][lg]) return 0;
     HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int i = 0; i < (big ? GRAPH_CYCLES : 1) && rc == 0; ++i) rc = enqueue_cycle(E, measure);
+    for (int i = 0; i < (1 << lg) && rc == 0; ++i) rc = enqueue_cycle(E, measure);
     hipGraph_t cap = nullptr;
     const hipError_t e2 = hipStreamEndCapture(E.stream, &cap);
     if (rc) {
@@ -488,7 +492,7 @@ int build_graph(Engine &E, int measure, int big)
     const hipError_t e3 = hipGraphInstantiate(&ex, cap, nullptr, nullptr, 0);
     (void)hipGraphDestroy(cap);
     HIPCHK(e3);
-    E.graph[measure][big] = ex;
+    E.graph[measure][lg] = ex;
     return 0;
 }
 
@@ -846,16 +850,19 @@ int run_cycles(Engine &E, int phase, int ncycles, int begin)
     int done = 0;
     if (!E.tuned && E.autotune && !E.timing && (rc = tune(E, measure, ncycles, done))) return rc;
     const bool graph = E.use_graph && !E.timing;
-    if (graph && ncycles - done >= GRAPH_CYCLES) {
-        if ((rc = build_graph(E, measure, 1))) return rc;
-        for (; done + GRAPH_CYCLES <= ncycles; done += GRAPH_CYCLES)
-            HIPCHK(hipGraphLaunch(E.graph[measure][1], E.stream));
+    if (graph) {
+        for (int lg = GRAPH_SIZES - 1; lg >= 0; --lg) {
+            const int n = 1 << lg;
+            if (ncycles - done < n) continue;
+            if ((rc = build_graph(E, measure, lg))) return rc;
+            do {                                  // 16 as often as it fits, then each smaller size at most once
+                HIPCHK(hipGraphLaunch(E.graph[measure][lg], E.stream));
+                done += n;
+            } while (lg == GRAPH_SIZES - 1 && ncycles - done >= n);
+        }
     }
-    if (graph && done < ncycles && (rc = build_graph(E, measure, 0))) return rc;
-    for (; done < ncycles; ++done) {
-        if (graph) HIPCHK(hipGraphLaunch(E.graph[measure][0], E.stream));
-        else if ((rc = enqueue_cycle(E, measure))) return rc;
-    }
+    for (; done < ncycles; ++done)
+        if ((rc = enqueue_cycle(E, measure))) return rc;
     E.cycles_done += (uint64_t)ncycles;
     if (E.st.fuse_locnem && ncycles > 0) {
         // accumulate the state the last measuring cycle left behind

@@ -1013,6 +1013,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     const int s0 = start + 2 * tid;
     const bool pair0 = !st.local && swaps && s0 < nrep - 1;
     const double b_lo = pair0 ? st.beta[sb + s0] : 0.0, b_hi = pair0 ? st.beta[sb + s0 + 1] : 0.0;
+    const uint32_t key1 = pair0 ? (uint32_t)st.confs[conf].iconf : 0u;   // the swap draws' key word
     const bool slot0 = measure && !st.local && tid < nrep;
     double S0[NSUMS], best0 = 0.0, hmin0 = 0.0, hmax0 = 0.0, hbin0 = 0.0;
     if (slot0) {
@@ -1048,7 +1049,7 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
             const double blo = s == s0 ? b_lo : st.beta[sb + s], bhi = s == s0 ? b_hi : st.beta[sb + s + 1];
             const double delta = (blo - bhi) * (et[rlo] - et[rhi]);
             const U4 rn = philox4x32_10((uint32_t)s, 0x80000000u, (uint32_t)cycle, (uint32_t)(cycle >> 32),
-                                        st.seed, (uint32_t)st.confs[conf].iconf);
+                                        st.seed, key1);
             const double u = (double)rn.x * 0x1p-32;
             if (delta >= 0.0 || u < exp64(delta)) {
                 atomicAdd(reinterpret_cast<unsigned long long *>(st.acc + pair_base + s), 1ull);

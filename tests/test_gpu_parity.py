@@ -167,13 +167,16 @@ def test_resident_and_global_paths_agree():
 
 
 def test_graph_and_eager_schedules_agree():
+    """Replayed graphs (16, 8, 4, 2 and 1 cycles: 31 = all five, 21 = 16 + 4
+    + 1 continuing the phase) give the eager schedule's results."""
     say("case graph vs eager")
     outs = []
     for flags in (0, J.FLAG_NO_GRAPH):
         g = J.make_params(8, 8, nreplica=4, nconf=2, flags=flags | LOCNEM, seed=SEED)
         eng, _ = build_engine([g], "hot")
         eng.run(J.PHASE_EQUIL, 3)
-        eng.run(J.PHASE_MEAS, 4)
+        eng.run(J.PHASE_MEAS, 31)
+        eng.run_cycles(J.PHASE_MEAS, 21)
         outs.append([eng.spins(c) for c in range(2)] + [eng.fetch(c)["basefields"] for c in range(2)])
     for a, b in zip(*outs):
         assert np.array_equal(a, b)

@@ -44,7 +44,7 @@ CASES = [
 
 
 def test_noalloc_matches_run_cycles():
-    neq, nmess = 7, 19           # 19 = one 16-cycle graph and three single-cycle graphs on the other path
+    neq, nmess = 7, 19           # 19 = graphs of 16, 2 and 1 cycles on the other path
     for label, shape, prec, starts in CASES:
         say(f"case noalloc: {label}")
         groups, iconf = [], 1
