@@ -105,6 +105,12 @@ bool bits_slab_fits(int32_t L, int32_t H, int32_t slabs, int32_t nblocks, bool s
 size_t bits_slab_x_words(int32_t nlat);     // slab_x words for nlat lattices (L <= 96)
 // nt in {256, 512, 576, 1024} for every L (and L = 0, mixed).
 bool resident_nt_valid(int32_t L, int32_t nt);
+// The resident kernel with NT threads per block for lattices of size L (0: a
+// launch of several sizes), FP64 shadow or not, H = L for every lattice
+// (cubic) or not; defined in kernels_res<NT>.hip.
+using ResKernel = void (*)(DeviceState, int32_t, int32_t);
+template <int NT>
+ResKernel resident_kernels(int32_t L, bool shadow, bool cubic);
 size_t resident_lds_limit();
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).
 hipError_t setup_resident(size_t max_lds);
