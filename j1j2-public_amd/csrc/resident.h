@@ -353,6 +353,30 @@ __device__ __forceinline__ NbPtr nb_shift(const NbPtr &P, int d)
 // (the low 16 bits of a site's word from the call with LO_CTR, drawn only when
 // the high ones cannot decide); P holds the LDS addresses of the word and its
 // neighbours.
+// The f16 views of the x neighbours straight from the two words of the
+// other-parity half-row (A: the same word, B: the one before (HX = 0) or
+// after (HX = 1)): the shifted neighbour word is [B3 A0 A1 A2] (HX = 0) or
+// [A1 A2 A3 B0] (HX = 1), so its even (HX = 0) or odd (HX = 1) view is one
+// byte permute of (A, B), and its other view is the unshifted word's other
+// view (f16_odd([B3 A0 A1 A2]) = f16_even(A), f16_even([A1 A2 A3 B0]) =
+// f16_odd(A)).
+template <int HX>
+__device__ __forceinline__ void x_views(uint32_t A, uint32_t B, uint32_t &xpe, uint32_t &xpo, uint32_t &xme,
+                                        uint32_t &xmo)
+{
+    if constexpr (HX == 0) {
+        xpe = f16_even(A);
+        xpo = f16_odd(A);
+        xme = __builtin_amdgcn_perm(A, B, 0x050c030cu);
+        xmo = xpe;
+    } else {
+        xme = f16_even(A);
+        xmo = f16_odd(A);
+        xpo = __builtin_amdgcn_perm(B, A, 0x040c020cu);
+        xpe = xmo;
+    }
+}
+
 template <int WPH, int HX, bool SHADOW, int ODD>
 __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int own,
                                            uint32_t m, int w, const NbPtr &P, const U4 &hi,
@@ -362,14 +386,16 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
     const uint32_t S = *P.own;
     const uint32_t YP = *P.yp, YM = *P.ym;
     const uint32_t ZP = *P.zp, ZM = *P.zm;
-    uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
-    x_from<HX>(*P.xa, *P.xb, Xp0, Xm0);
-    x_from<HX>(*P.d1a, *P.d1b, Xp1, Xm1);
-    x_from<HX>(*P.d2a, *P.d2b, Xp2, Xm2);
-    const uint32_t D = (Xp1 + Xm1) + (Xp2 + Xm2);     // bytewise 4 * (0..4)
+    const uint32_t XA = *P.xa, XB = *P.xb, D1A = *P.d1a, D1B = *P.d1b, D2A = *P.d2a, D2B = *P.d2b;
+    // diagonals: the two rows' words summed first (bytes 0, 4, 8: no carries),
+    // so the x shift of the sum is one align
+    uint32_t Dp, Dm;
+    x_from<HX>(D1A + D2A, D1B + D2B, Dp, Dm);
+    const uint32_t D = Dp + Dm;                         // bytewise 4 * (0..4)
     const uint32_t Z = ZP + ZM;                         // bytewise 4 * (0..2)
     const uint32_t de = lo_even(D), dod = lo_odd(D), ze = lo_even(Z), zod = lo_odd(Z);
-    const uint32_t xpe = f16_even(Xp0), xpo = f16_odd(Xp0), xme = f16_even(Xm0), xmo = f16_odd(Xm0);
+    uint32_t xpe, xpo, xme, xmo;
+    x_views<HX>(XA, XB, xpe, xpo, xme, xmo);
     const uint32_t ype = f16_even(YP), ypo = f16_odd(YP), yme = f16_even(YM), ymo = f16_odd(YM);
     // 32-bit byte offset from a uniform base, formed here: hidden from the
     // optimiser, which would otherwise hoist every word's five 64-bit plane
@@ -423,6 +449,10 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
             const int mm = 4 * w + j;                         // position in the half-row
             const int mo_p = HX ? ((mm == HALF - 1) ? 0 : mm + 1) : mm;
             const int mo_m = HX ? mm : ((mm == 0) ? HALF - 1 : mm - 1);
+            uint32_t Xp0, Xm0, Xp1, Xm1, Xp2, Xm2;
+            x_from<HX>(XA, XB, Xp0, Xm0);
+            x_from<HX>(D1A, D1B, Xp1, Xm1);
+            x_from<HX>(D2A, D2B, Xp2, Xm2);
             const double sv[10] = {pm1(lds_bit(YP, j)), pm1(lds_bit(YM, j)), pm1(lds_bit(Xp0, j)),
                                    pm1(lds_bit(Xm0, j)), pm1(lds_bit(Xp1, j)), pm1(lds_bit(Xm1, j)),
                                    pm1(lds_bit(Xp2, j)), pm1(lds_bit(Xm2, j)), pm1(lds_bit(ZP, j)),
