@@ -433,7 +433,7 @@ def main():
     if args.flips:
         fl = time_form(J, J.FP32_BATCHED, args.steps, args.warmup, kernel_timing=False, flags=J.FLAG_WRITENUMFLIPS)
         res["with_flip_counts"] = {"value": fl["rate"], "sweep_path": J.PATH_NAMES[fl["path"]],
-                                   "over_value": fl["rate"] / fp32["rate"]}
+                                   "launch_plan": fl["plan"], "over_value": fl["rate"] / fp32["rate"]}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline()
     print(json.dumps(res), flush=True)

@@ -677,7 +677,9 @@ std::string tune_key(const Engine &E)
 {
     hipDeviceProp_t prop{};
     (void)hipGetDeviceProperties(&prop, E.device);
-    std::string k = std::string(prop.gcnArchName) + (E.shadow ? " shadow" : "");
+    // flip counts double a resident block's LDS (fewer blocks per CU), so a
+    // choice timed without them does not carry over
+    std::string k = std::string(prop.gcnArchName) + (E.shadow ? " shadow" : "") + (E.st.flips ? " flips" : "");
     // the configs x replicas split fixes the work lists' shapes (their XCD
     // balance and padding), so it is part of the shape a choice was timed on
     for (const Group &G : E.groups)
@@ -1218,6 +1220,10 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     st.local = c0.sampler == J1J2_SAMPLER_LOCAL ? 1 : 0;
     st.seed = (uint32_t)c0.seed;
     st.fuse_locnem = (E->locnem && E->all_resident) ? 1 : 0;
+    st.locsnap = nullptr;
+    if (st.fuse_locnem && std::any_of(E->groups.begin(), E->groups.end(), [](const Group &G) { return G.kind == 1; }) &&
+        (rc = E->alloc(st.locsnap, E->nsites / 32)))
+        return rc;
     {
         // work lists of the resident launches: per resident group, and over all lattices
         std::vector<int32_t> order;

@@ -55,6 +55,8 @@ struct DeviceState {
     int64_t *locsamples;       // nconf
     int32_t *locnem_pending;   // nconf: the last cycle measured; its state is still to be accumulated
     int32_t fuse_locnem;       // 1: the resident sweep accumulates pending local-nematic sums
+    uint32_t *locsnap;         // nsites / 32 words, or null: per byte-layout config (at site_off / 32) the
+                               // packed spins of the replica a measuring PT step left at slot PTMEASINDEX
     double *series;            // LOCAL series
     int64_t *series_len;       // nconf
     int64_t series_cap;

@@ -381,6 +381,19 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
             if (st.local) st.series_len[conf] = slen + 1;
         }
     }
+    // The replica now at slot PTMEASINDEX: a copy of its packed spins (byte-
+    // layout configs), from which every block of the config's next sweep
+    // launch adds its share to the local nematic sums (the sweep overwrites
+    // the spins themselves)
+    if (swaps && measure && st.fuse_locnem && st.locsnap) {
+        const LatDesc d = st.lats[lat0 + rat[st.ptmeasindex - 1]];
+        if (d.packed == 1) {
+            const uint2 *src = reinterpret_cast<const uint2 *>(st.spins + d.spin_off);
+            uint2 *dst = reinterpret_cast<uint2 *>(st.locsnap + d.site_off / 32);
+#pragma unroll 8
+            for (int i = tid; i < d.N / 64; i += PT_THREADS) dst[i] = src[i];
+        }
+    }
     if (swaps && tid == 0) {
         if (st.fuse_locnem) st.locnem_pending[conf] = measure ? 1 : 0;
         // one tick per block (nconf per cycle, see clock_cycle): the block's

@@ -437,22 +437,38 @@ __device__ __forceinline__ void bit_locnem(const uint32_t *lds32, int c, int H, 
     if (tid >= TPL) return;
     const int yh = tid / G, j = tid - yh * G;
     const int nq = HW * H * G;
-    for (int zl = 0; zl < sh.nz; ++zl) {
-        const int zg = sh.z0 + zl;
-        const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
-        const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, group_pos<L>(j));
-        const uint32_t S = spread4(s.S);
-        const uint32_t ax = (S ^ s.xp) + (S ^ s.xm), ay = (S ^ s.yp) + (S ^ s.ym);
-        const int dw = c * nq + zg * TPL + tid;
-        int4 ls = reinterpret_cast<int4 *>(locspin)[dw], lv = reinterpret_cast<int4 *>(locnem)[dw];
-        int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
+    int4 *ls4 = reinterpret_cast<int4 *>(locspin), *lv4 = reinterpret_cast<int4 *>(locnem);
+    // layers in batches of K whose accumulators (in HBM) are all requested
+    // before the first is added to: one memory round trip per batch
+    constexpr int K = 4;
+    for (int zb = 0; zb < sh.nz; zb += K) {
+        int4 ls[K], lv[K];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            lsp[k] += (int)nib_bit(s.S, k) * 2 - 1;
-            lnp[k] += 2 * ((int)((ay >> (8 * k + 2)) & 0xffu) - (int)((ax >> (8 * k + 2)) & 0xffu));
+        for (int q = 0; q < K; ++q)
+            if (zb + q < sh.nz) {
+                const int dw = c * nq + (sh.z0 + zb + q) * TPL + tid;
+                ls[q] = ls4[dw];
+                lv[q] = lv4[dw];
+            }
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const int zl = zb + q;
+            if (zl >= sh.nz) break;
+            const int zg = sh.z0 + zl;
+            const BitNb n = bit_nb<HW, SLAB>(c, zg, zl, yh, H);
+            const BitSites s = bit_sites<HW, HX>(lds32, c, sh.NU, n, group_pos<L>(j));
+            const uint32_t S = spread4(s.S);
+            const uint32_t ax = (S ^ s.xp) + (S ^ s.xm), ay = (S ^ s.yp) + (S ^ s.ym);
+            int *lsp = reinterpret_cast<int *>(&ls[q]), *lnp = reinterpret_cast<int *>(&lv[q]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                lsp[k] += (int)nib_bit(s.S, k) * 2 - 1;
+                lnp[k] += 2 * ((int)((ay >> (8 * k + 2)) & 0xffu) - (int)((ax >> (8 * k + 2)) & 0xffu));
+            }
+            const int dw = c * nq + zg * TPL + tid;
+            ls4[dw] = ls[q];
+            lv4[dw] = lv[q];
         }
-        reinterpret_cast<int4 *>(locspin)[dw] = ls;
-        reinterpret_cast<int4 *>(locnem)[dw] = lv;
     }
 }
 

@@ -92,45 +92,70 @@ __device__ __forceinline__ void x_pair(const uint32_t *base, int uoff, int w, ui
     }
 }
 
-// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) over the units of
-// one lattice in the colour-plane layout: l_i = 2 (a(+y) + a(-y) - a(+x) -
-// a(-x)), a = 1 if the bond is anti-aligned.  A site's spin is bit SH of its
-// byte: SH = 0 in global memory, 2 in the resident sweep's LDS copy; PK = 1:
-// global memory holds one bit per site (rd_word).
+// Accumulate_local_nematic_average (J1J2_6tv.f90:1009-1025) for one word
+// (four sites) of plane c in the colour-plane layout: l_i = 2 (a(+y) + a(-y) -
+// a(+x) - a(-x)), a = 1 if the bond is anti-aligned.  A site's spin is bit SH
+// of its byte: SH = 0 in global memory, 2 in the resident sweep's LDS copy;
+// PK = 1: global memory holds one bit per site (rd_word).
 template <int WPH, int HX, int SH, int PK = 0>
-__device__ __forceinline__ void locnem_plane(const uint32_t *sp32, int c, int H, int t0, int stride,
-                                             int32_t *locspin, int32_t *locnem)
+__device__ __forceinline__ void locnem_word(const uint32_t *sp32, int c, int H, int t, int4 &ls, int4 &lv)
 {
     const int nq = 4 * WPH * WPH * H;
-    for (int t = t0; t < nq; t += stride) {
-        const int u = t / WPH, w = t - u * WPH;
-        const UnitGeo g = unit_geo<WPH>(u, c, H);
-        const int py = (c ^ 2) * nq, dw = c * nq + t;
-        const uint32_t S = rd_word<PK>(sp32, dw);
-        const uint32_t YP = rd_word<PK>(sp32, py + g.uyp * WPH + w), YM = rd_word<PK>(sp32, py + g.uym * WPH + w);
-        uint32_t Xp, Xm;
-        x_pair<WPH, HX, PK>(sp32, (c ^ 1) * nq + u * WPH, w, Xp, Xm);
-        const uint32_t ax = (S ^ Xp) + (S ^ Xm), ay = (S ^ YP) + (S ^ YM);
-        int4 ls = reinterpret_cast<int4 *>(locspin)[dw], lv = reinterpret_cast<int4 *>(locnem)[dw];
-        int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
+    const int u = t / WPH, w = t - u * WPH;
+    const UnitGeo g = unit_geo<WPH>(u, c, H);
+    const int py = (c ^ 2) * nq;
+    const uint32_t S = rd_word<PK>(sp32, c * nq + t);
+    const uint32_t YP = rd_word<PK>(sp32, py + g.uyp * WPH + w), YM = rd_word<PK>(sp32, py + g.uym * WPH + w);
+    uint32_t Xp, Xm;
+    x_pair<WPH, HX, PK>(sp32, (c ^ 1) * nq + u * WPH, w, Xp, Xm);
+    const uint32_t ax = (S ^ Xp) + (S ^ Xm), ay = (S ^ YP) + (S ^ YM);
+    int *lsp = reinterpret_cast<int *>(&ls), *lnp = reinterpret_cast<int *>(&lv);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            lsp[j] += (int)((S >> (8 * j + SH)) & 1u) * 2 - 1;
-            lnp[j] += 2 * ((int)((ay >> (8 * j + SH)) & 0xffu) - (int)((ax >> (8 * j + SH)) & 0xffu));
-        }
-        reinterpret_cast<int4 *>(locspin)[dw] = ls;
-        reinterpret_cast<int4 *>(locnem)[dw] = lv;
+    for (int j = 0; j < 4; ++j) {
+        lsp[j] += (int)((S >> (8 * j + SH)) & 1u) * 2 - 1;
+        lnp[j] += 2 * ((int)((ay >> (8 * j + SH)) & 0xffu) - (int)((ax >> (8 * j + SH)) & 0xffu));
     }
 }
 
+// The same over the words [gb, ge) of a lattice (word g = c nq + t: plane c,
+// word t), a thread taking g = gb + t0, gb + t0 + stride, ... in batches of K
+// words whose accumulators (in HBM) are all requested before the first is
+// added to: one memory round trip per batch instead of one per word.
+template <int WPH, int SH, int PK = 0, int K = 4>
+__device__ __forceinline__ void locnem_range(const uint32_t *sp32, int H, int gb, int ge, int t0, int stride,
+                                             int32_t *locspin, int32_t *locnem)
+{
+    const int nq = 4 * WPH * WPH * H;
+    int4 *ls4 = reinterpret_cast<int4 *>(locspin), *lv4 = reinterpret_cast<int4 *>(locnem);
+    for (int g0 = gb + t0; g0 < ge; g0 += K * stride) {
+        int4 ls[K], lv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int g = g0 + k * stride;
+            if (g < ge) {
+                ls[k] = ls4[g];
+                lv[k] = lv4[g];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int g = g0 + k * stride;
+            if (g >= ge) break;
+            const int c = g / nq, t = g - c * nq;
+            if (c & 1) locnem_word<WPH, 1, SH, PK>(sp32, c, H, t, ls[k], lv[k]);
+            else locnem_word<WPH, 0, SH, PK>(sp32, c, H, t, ls[k], lv[k]);
+            ls4[g] = ls[k];
+            lv4[g] = lv[k];
+        }
+    }
+}
+
+// Every word of a lattice.
 template <int WPH, int SH, int PK = 0>
 __device__ __forceinline__ void locnem_units(const uint32_t *sp32, int H, int t0, int stride, int32_t *locspin,
                                              int32_t *locnem)
 {
-    for (int c = 0; c < 4; ++c) {
-        if (c & 1) locnem_plane<WPH, 1, SH, PK>(sp32, c, H, t0, stride, locspin, locnem);
-        else locnem_plane<WPH, 0, SH, PK>(sp32, c, H, t0, stride, locspin, locnem);
-    }
+    locnem_range<WPH, SH, PK>(sp32, H, 0, 16 * WPH * WPH * H, t0, stride, locspin, locnem);
 }
 
 // Per-thread partial sums of the integer observables in int32 (a thread covers
@@ -594,6 +619,23 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
+    // The last measuring cycle left a replica of this config at slot
+    // PTMEASINDEX, and its PT step kept that replica's packed spins: every
+    // block of the config adds its share of the words to the local nematic
+    // sums (J1J2_6tv.f90:1009-1025), reading the copy from LDS (PK = 1, before
+    // the lattice takes the LDS), so no block carries the whole lattice's
+    // accumulator traffic.
+    if (st.fuse_locnem && st.locsnap && st.locnem_pending[d.conf]) {
+        const uint32_t *snap = st.locsnap + d.site_off / 32;
+#pragma unroll 4
+        for (int i = tid; i < N / 32; i += NT) lds32[i] = snap[i];
+        __syncthreads();
+        const int nw = N / 4, R = st.confs[d.conf].nrep;
+        const int gb = (int)((int64_t)nw * d.rep / R), ge = (int)((int64_t)nw * (d.rep + 1) / R);
+        locnem_range<WPH, 0, 1>(lds32, H, gb, ge, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
+        if (tid == 0 && d.rep == 0) st.locsamples[d.conf] += 1;
+        __syncthreads();                                // the copy's LDS becomes the lattice's
+    }
     // bits -> bytes 4b in LDS (32 sites per word); a thread's words are all
     // requested before the first is unpacked (one memory round trip, not four)
     constexpr int LDU = 4;
@@ -641,13 +683,6 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     const uint64_t S0 = clock_cycle(st) * (uint64_t)st.ptswapever;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
-    // the previous cycle measured and left this replica at slot PTMEASINDEX:
-    // accumulate its local nematic sums from the state it ended in
-    if (st.fuse_locnem && slot == st.ptmeasindex - 1 && st.locnem_pending[d.conf]) {
-        locnem_units<WPH, 2>(lds32, H, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
-        if (tid == 0) st.locsamples[d.conf] += 1;
-        __syncthreads();   // every read of the unswept state precedes the first colour pass
-    }
 
     for (int k = 0; k < st.ptswapever; ++k) {
         const uint64_t sweep = S0 + (uint64_t)k;
@@ -667,16 +702,28 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     }
     if (counting) {
         // confnumflips(slot, site) += this launch's counts (internal site order)
+        // (in batches of four words, the counters of a batch all requested
+        // before the first is added to)
         int4 *fl = reinterpret_cast<int4 *>(st.flips + cd.flip_off + (int64_t)slot * N);
-        for (int i = tid; i < N / 4; i += NT) {
-            const uint32_t n4 = lds32[N / 4 + i];
-            if (!n4) continue;
-            int4 g = fl[i];
-            g.x += (int)(n4 & 0xFFu);
-            g.y += (int)((n4 >> 8) & 0xFFu);
-            g.z += (int)((n4 >> 16) & 0xFFu);
-            g.w += (int)(n4 >> 24);
-            fl[i] = g;
+        constexpr int KF = 4;
+        for (int i0 = tid; i0 < N / 4; i0 += KF * NT) {
+            int4 g[KF];
+            uint32_t n4[KF];
+#pragma unroll
+            for (int q = 0; q < KF; ++q) {
+                const int i = i0 + q * NT;
+                n4[q] = i < N / 4 ? lds32[N / 4 + i] : 0u;
+                if (n4[q]) g[q] = fl[i];
+            }
+#pragma unroll
+            for (int q = 0; q < KF; ++q) {
+                if (!n4[q]) continue;
+                g[q].x += (int)(n4[q] & 0xFFu);
+                g[q].y += (int)((n4[q] >> 8) & 0xFFu);
+                g[q].z += (int)((n4[q] >> 16) & 0xFFu);
+                g[q].w += (int)(n4[q] >> 24);
+                fl[i0 + q * NT] = g[q];
+            }
         }
     }
 
