@@ -388,10 +388,23 @@ __global__ __launch_bounds__(PT_THREADS) void k_pt_step(DeviceState st, int32_t 
     if (swaps && measure && st.fuse_locnem && st.locsnap) {
         const LatDesc d = st.lats[lat0 + rat[st.ptmeasindex - 1]];
         if (d.packed == 1) {
+            // eight words per thread requested before the first is stored
             const uint2 *src = reinterpret_cast<const uint2 *>(st.spins + d.spin_off);
             uint2 *dst = reinterpret_cast<uint2 *>(st.locsnap + d.site_off / 32);
-#pragma unroll 8
-            for (int i = tid; i < d.N / 64; i += PT_THREADS) dst[i] = src[i];
+            constexpr int CB = 8;
+            for (int i0 = tid; i0 < d.N / 64; i0 += CB * PT_THREADS) {
+                uint2 v[CB];
+#pragma unroll
+                for (int q = 0; q < CB; ++q) {
+                    const int i = i0 + q * PT_THREADS;
+                    if (i < d.N / 64) v[q] = src[i];
+                }
+#pragma unroll
+                for (int q = 0; q < CB; ++q) {
+                    const int i = i0 + q * PT_THREADS;
+                    if (i < d.N / 64) dst[i] = v[q];
+                }
+            }
         }
     }
     if (swaps && tid == 0) {

@@ -619,32 +619,71 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
+    // bits -> bytes 4b in LDS (32 sites per word); a thread's words are all
+    // requested before the first is unpacked (one memory round trip, not
+    // four), the first batch before anything else
+    constexpr int LDU = 4;
+    const uint32_t *gsp32 = reinterpret_cast<const uint32_t *>(gsp);
+    uint32_t W0[LDU];
+#pragma unroll
+    for (int q = 0; q < LDU; ++q) {
+        const int i = tid + q * NT;
+        W0[q] = i < N / 32 ? gsp32[i] : 0u;
+    }
     // The last measuring cycle left a replica of this config at slot
     // PTMEASINDEX, and its PT step kept that replica's packed spins: every
     // block of the config adds its share of the words to the local nematic
     // sums (J1J2_6tv.f90:1009-1025), reading the copy from LDS (PK = 1, before
     // the lattice takes the LDS), so no block carries the whole lattice's
-    // accumulator traffic.
+    // accumulator traffic.  The first batch of accumulators is requested with
+    // the copy and the lattice words above (one round trip for all three).
     if (st.fuse_locnem && st.locsnap && st.locnem_pending[d.conf]) {
+        const int nw = N / 4, R = st.confs[d.conf].nrep;
+        const int gb = (int)((int64_t)nw * d.rep / R), ge = (int)((int64_t)nw * (d.rep + 1) / R);
+        int4 *ls4 = reinterpret_cast<int4 *>(st.locspin + d.site_off);
+        int4 *lv4 = reinterpret_cast<int4 *>(st.locnem + d.site_off);
+        constexpr int K = 4;
+        int4 ls[K], lv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int g = gb + tid + k * NT;
+            if (g < ge) {
+                ls[k] = ls4[g];
+                lv[k] = lv4[g];
+            }
+        }
         const uint32_t *snap = st.locsnap + d.site_off / 32;
 #pragma unroll 4
         for (int i = tid; i < N / 32; i += NT) lds32[i] = snap[i];
         __syncthreads();
-        const int nw = N / 4, R = st.confs[d.conf].nrep;
-        const int gb = (int)((int64_t)nw * d.rep / R), ge = (int)((int64_t)nw * (d.rep + 1) / R);
-        locnem_range<WPH, 0, 1>(lds32, H, gb, ge, tid, NT, st.locspin + d.site_off, st.locnem + d.site_off);
+        const int nq = N / 16;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int g = gb + tid + k * NT;
+            if (g >= ge) break;
+            const int c = g / nq, t = g - c * nq;
+            if (c & 1) locnem_word<WPH, 1, 0, 1>(lds32, c, H, t, ls[k], lv[k]);
+            else locnem_word<WPH, 0, 0, 1>(lds32, c, H, t, ls[k], lv[k]);
+            ls4[g] = ls[k];
+            lv4[g] = lv[k];
+        }
+        // the rest of the share (blocks of configs with few replicas)
+        locnem_range<WPH, 0, 1>(lds32, H, gb + K * NT, ge, tid, NT, st.locspin + d.site_off,
+                                st.locnem + d.site_off);
         if (tid == 0 && d.rep == 0) st.locsamples[d.conf] += 1;
         __syncthreads();                                // the copy's LDS becomes the lattice's
     }
-    // bits -> bytes 4b in LDS (32 sites per word); a thread's words are all
-    // requested before the first is unpacked (one memory round trip, not four)
-    constexpr int LDU = 4;
     for (int i0 = tid; i0 < N / 32; i0 += LDU * NT) {
         uint32_t W[LDU];
+        if (i0 == tid) {
 #pragma unroll
-        for (int q = 0; q < LDU; ++q) {
-            const int i = i0 + q * NT;
-            W[q] = i < N / 32 ? reinterpret_cast<const uint32_t *>(gsp)[i] : 0u;
+            for (int q = 0; q < LDU; ++q) W[q] = W0[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < LDU; ++q) {
+                const int i = i0 + q * NT;
+                W[q] = i < N / 32 ? gsp32[i] : 0u;
+            }
         }
 #pragma unroll
         for (int q = 0; q < LDU; ++q) {
