@@ -69,6 +69,10 @@ struct ResLaunch {
     int32_t ord0, nord, L, max_n, nt, nlat;
     int32_t kind = 1, H = 0;   // kind 2: bit layout (nt fixed by L)
     int32_t slabs = 1;         // kind 2: z-slabs per lattice (one block each)
+    bool bundled = false;      // work-list entries are bundles (the launch over several sizes)
+    // the L the kernel is picked by: a bundled launch takes the kernel over
+    // several sizes (L = 0, its H then -1 when every lattice is cubic)
+    int32_t kernel_L() const { return bundled ? 0 : L; }
 };
 
 // How a cycle's resident sweeps are launched.  The result of a cycle does not
@@ -82,7 +86,7 @@ struct Plan {
         std::string d = concurrent ? "concurrent" : "sequential";
         for (const ResLaunch &r : launches)
             d += " [L" + std::to_string(r.L) + " nt" + std::to_string(r.nt) + " x" + std::to_string(r.nlat) +
-                 (r.slabs > 1 ? " s" + std::to_string(r.slabs) : std::string()) + "]";
+                 (r.slabs > 1 ? " s" + std::to_string(r.slabs) : std::string()) + (r.bundled ? " bundled" : "") + "]";
         return d;
     }
 };
@@ -110,6 +114,8 @@ struct Engine {
     Plan plan;                                 // resident launches of a cycle
     std::vector<OrderSeg> group_seg;           // work-list segment of each resident group
     OrderSeg all_seg;                          // of all lattices (when all are resident)
+    OrderSeg all_seg_b;                        // the same in bundles (1024-thread FP32 launch)
+    int32_t all_max_b = 0;                     // the bundles' largest LDS lattice bytes (without flip counts)
     int32_t order_len = 0;                     // entries of the device work list (st.order)
     bool tuned = false, autotune = true;
     std::string tune_key;
@@ -405,8 +411,8 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
                     const ResLaunch &L = pl.launches[i];
                     if ((r = hipStreamWaitEvent(e.side[i], e.fork_ev, 0)) != hipSuccess) break;
                     r = L.kind == 2 ? launch_sweep_bits(e.st, L.ord0, L.nord, L.L, L.H, L.slabs, e.shadow, e.side[i])
-                                    : launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.L, L.H, L.nt, e.shadow,
-                                                            e.side[i]);
+                                    : launch_sweep_resident(e.st, L.ord0, L.nord, L.max_n, L.kernel_L(), L.H, L.nt,
+                                                            e.shadow, e.side[i]);
                     if (r != hipSuccess) break;
                     if ((r = hipEventRecord(e.join_ev[i], e.side[i])) != hipSuccess) break;
                     r = hipStreamWaitEvent(e.stream, e.join_ev[i], 0);
@@ -419,8 +425,8 @@ int enqueue_cycle(Engine &E, int mode, const Plan *plan = nullptr, Probe *probe 
                 auto fn = [](Engine &e, const void *a) {
                     const ResLaunch &r = *static_cast<const ResLaunch *>(a);
                     return r.kind == 2 ? launch_sweep_bits(e.st, r.ord0, r.nord, r.L, r.H, r.slabs, e.shadow, e.stream)
-                                       : launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.L, r.H, r.nt,
-                                                               e.shadow, e.stream);
+                                       : launch_sweep_resident(e.st, r.ord0, r.nord, r.max_n, r.kernel_L(), r.H,
+                                                               r.nt, e.shadow, e.stream);
                 };
                 const size_t i = (size_t)(&L - P.launches.data());
                 if (probe && i < probe->per.size()) HIPCHK(hipEventRecord(probe->per[i].first, E.stream));
@@ -539,6 +545,12 @@ ResLaunch group_launch(const Engine &E, int g, int32_t nt, int32_t slabs = 1)
 
 ResLaunch all_launch(const Engine &E, int32_t nt)
 {
+    if (nt == 1024 && E.all_seg_b.nord > 0) {                  // bundles of small lattices
+        ResLaunch r{E.all_seg_b.ord0, E.all_seg_b.nord, E.uniform_L, E.all_max_b, nt, E.st.nlat};
+        r.H = E.all_cubic ? -1 : 0;
+        r.bundled = true;
+        return r;
+    }
     ResLaunch r{E.all_seg.ord0, E.all_seg.nord, E.uniform_L, E.max_n, nt, E.st.nlat};
     r.H = E.uniform_L ? E.uniform_H : (E.all_cubic ? -1 : 0);
     return r;
@@ -583,7 +595,9 @@ Plan default_plan(const Engine &E)
 // accumulated work, then interleaved so that block 8k + x is XCD x's k-th
 // lattice (-1 pads the shorter XCD lists).  slabs > 1: one entry
 // BIT_SLAB_MAX * lattice + s per slab s, a lattice's slabs together on one XCD.
-std::vector<int32_t> make_order(const Engine &E, int c0, int c1, int slabs = 1)
+// bundle: entries of up to res_bundle(L) lattices of one config (first lattice
+// plus (count - 1) << BUNDLE_SHIFT), for the 1024-thread launch over all sizes.
+std::vector<int32_t> make_order(const Engine &E, int c0, int c1, int slabs = 1, bool bundle = false)
 {
     std::vector<int> cs;
     for (int c = c0; c < c1; ++c) cs.push_back(c);
@@ -593,11 +607,12 @@ std::vector<int32_t> make_order(const Engine &E, int c0, int c1, int slabs = 1)
     for (int c : cs) {
         const int x = (int)(std::min_element(work.begin(), work.end()) - work.begin());
         const ConfDesc &cd = E.confs[c];
-        for (int r = 0; r < cd.nrep; ++r) {
+        const int G = bundle ? res_bundle(cd.L) : 1;
+        for (int r = 0; r < cd.nrep; r += G) {
             if (slabs > 1)
                 for (int q = 0; q < slabs; ++q) bins[x].push_back((cd.lat0 + r) * BIT_SLAB_MAX + q);
             else
-                bins[x].push_back(cd.lat0 + r);
+                bins[x].push_back((cd.lat0 + r) | ((std::min(G, cd.nrep - r) - 1) << BUNDLE_SHIFT));
         }
         work[x] += (int64_t)cd.N * cd.nrep;
     }
@@ -918,7 +933,8 @@ int enqueue_cycle_noalloc(const Engine &E, const DeviceState &st, hipStream_t q,
     if ((rc = check_plan(E, E.plan))) return rc;
     for (const ResLaunch &r : E.plan.launches)
         HIPCHK(r.kind == 2 ? launch_sweep_bits(st, r.ord0, r.nord, r.L, r.H, r.slabs, E.shadow, q)
-                           : launch_sweep_resident(st, r.ord0, r.nord, r.max_n, r.L, r.H, r.nt, E.shadow, q));
+                           : launch_sweep_resident(st, r.ord0, r.nord, r.max_n, r.kernel_L(), r.H, r.nt, E.shadow,
+                                                   q));
     for (const Group &G : E.groups) {
         if (G.resident) continue;
         for (int k = 0; k < E.common.ptswapever; ++k)
@@ -1252,6 +1268,18 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
                     if (bits_slab_valid(G.p.L, G.p.H, S)) G.slab_seg[S] = add_slabs(G.conf0, G.conf0 + G.p.nconf, S);
         }
         if (E->all_byte) E->all_seg = add(0, (int)nconf);
+        // the same in bundles for the 1024-thread launch (FP32 without the shadow
+        // rule), when some size bundles and every lattice index fits an entry
+        bool bundles = false;
+        for (const Group &G : E->groups) bundles = bundles || res_bundle(G.p.L) > 1;
+        if (E->all_byte && !E->shadow && bundles && nlat <= BUNDLE_LAT_MASK) {
+            const std::vector<int32_t> seg = make_order(*E, 0, (int)nconf, 1, true);
+            E->all_seg_b.ord0 = (int32_t)order.size();
+            E->all_seg_b.nord = (int32_t)seg.size();
+            order.insert(order.end(), seg.begin(), seg.end());
+            for (const Group &G : E->groups) E->all_max_b = std::max(E->all_max_b, res_bundle(G.p.L) * G.N);
+            if ((size_t)E->all_max_b * (flips ? 2 : 1) > lds_limit) E->all_seg_b = OrderSeg{};   // no room
+        }
         // a forced slab count (2 or 4) only where it applies to every bit-layout group
         if (E->bit_slabs != 2 && E->bit_slabs != 4) E->bit_slabs = 1;
         for (const Group &G : E->groups)

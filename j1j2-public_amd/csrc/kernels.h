@@ -88,6 +88,14 @@ struct DeviceState {
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord,
                                  int32_t max_n, int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);
+// Bundles of the 1024-thread FP32 launch over lattices of several sizes: up to
+// res_bundle(L) lattices of one config per block, side by side in LDS (one
+// group of 1024 / res_bundle(L) threads each).  A work-list entry of that
+// launch holds the first lattice in its low BUNDLE_SHIFT bits and the count
+// minus one above them.
+constexpr int BUNDLE_SHIFT = 24;
+constexpr int32_t BUNDLE_LAT_MASK = (1 << BUNDLE_SHIFT) - 1;
+constexpr int res_bundle(int32_t L) { return L == 16 ? 8 : (L == 24 || L == 32) ? 2 : 1; }
 // Bytes of LDS the resident kernels add after the lattice (reduction scratch).
 constexpr size_t RESIDENT_SCRATCH_LDS = sizeof(int64_t) * (1024 / 64) * 8;
 // Bit-layout resident sweep (kernels_bits.hip): L % 8 == 0, 48 < L <= 96, one

@@ -287,10 +287,10 @@ constexpr bool unrolled_pairs()
 
 template <int WPH, int NT, int HX>
 __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const QPlanes &qp,
-                                             ObsFast &a)
+                                             ObsFast &a, int tid)
 {
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH, ZSTEP = fast_zs<WPH, NT>();
-    const int nq = PER_Z * H, tid = threadIdx.x;
+    const int nq = PER_Z * H;
     if (tid >= ZSTEP * PER_Z) return;
     const int tz = tid % PER_Z, w = tz % WPH;
     int z0 = tid / PER_Z;
@@ -531,11 +531,11 @@ __device__ __forceinline__ NbPtr odd_layer(const NbPtr &P, const uint32_t *yf, c
 // per pair of words (m = zp PER_Z + the word's index within its layer).
 template <int WPH, int NT, int C, bool SHADOW, int HC = 0>
 __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int H,
-                                            unsigned long long &sh_prop, unsigned long long &sh_mis)
+                                            unsigned long long &sh_prop, unsigned long long &sh_mis, int tid)
 {
     constexpr int HX = C & 1, PY = C >> 1;
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH;    // dwords per z-layer of a plane
-    const int nq = PER_Z * H, tid = threadIdx.x, npairs = H >> 1;
+    const int nq = PER_Z * H, npairs = H >> 1;
     if constexpr (pair_zs<WPH, NT>() >= 1) {
         // Each thread keeps its (y/2, w) and steps over layer pairs by ZP
         // pairs: every neighbour offset is constant except at the z boundary
@@ -610,12 +610,17 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
     }
 }
 
+// The sweeps of one lattice by a group of NT threads (tid: index in the
+// group; the whole block, or one of several groups that sweep a lattice each
+// in their own part of the LDS).  active = false: the group repeats the work
+// of another group's lattice in its own LDS (a bundle with fewer lattices
+// than groups) and writes nothing to global memory; it takes every barrier.
 template <int WPH, int NT, bool SHADOW, int HC = 0>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
-                                              uint32_t *lds32, int64_t (*red)[OBS_N])
+                                              uint32_t *lds32, int64_t (*red)[OBS_N], int tid = threadIdx.x,
+                                              bool active = true)
 {
     const int H = HC > 0 ? HC : d.H;
-    const int tid = threadIdx.x;
     const int N = 64 * WPH * WPH * H;
     uint8_t *gsp = st.spins + d.spin_off;
 
@@ -660,7 +665,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int g = gb + tid + k * NT;
-            if (g >= ge) break;
+            if (g >= ge || !active) break;
             const int c = g / nq, t = g - c * nq;
             if (c & 1) locnem_word<WPH, 1, 0, 1>(lds32, c, H, t, ls[k], lv[k]);
             else locnem_word<WPH, 0, 0, 1>(lds32, c, H, t, ls[k], lv[k]);
@@ -668,9 +673,11 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
             lv4[g] = lv[k];
         }
         // the rest of the share (blocks of configs with few replicas)
-        locnem_range<WPH, 0, 1>(lds32, H, gb + K * NT, ge, tid, NT, st.locspin + d.site_off,
-                                st.locnem + d.site_off);
-        if (tid == 0 && d.rep == 0) st.locsamples[d.conf] += 1;
+        if (active) {
+            locnem_range<WPH, 0, 1>(lds32, H, gb + K * NT, ge, tid, NT, st.locspin + d.site_off,
+                                    st.locnem + d.site_off);
+            if (tid == 0 && d.rep == 0) st.locsamples[d.conf] += 1;
+        }
         __syncthreads();                                // the copy's LDS becomes the lattice's
     }
     for (int i0 = tid; i0 < N / 32; i0 += LDU * NT) {
@@ -731,15 +738,15 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
             switch (c) {
-                case 0: colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
-                case 1: colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
-                case 2: colour_pass<WPH, NT, 2, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
-                default: colour_pass<WPH, NT, 3, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis); break;
+                case 0: colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
+                case 1: colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
+                case 2: colour_pass<WPH, NT, 2, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
+                default: colour_pass<WPH, NT, 3, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
             }
             __syncthreads();
         }
     }
-    if (counting) {
+    if (counting && active) {
         // confnumflips(slot, site) += this launch's counts (internal site order)
         // (in batches of four words, the counters of a batch all requested
         // before the first is added to)
@@ -772,8 +779,8 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     if constexpr (fast_zs<WPH, NT>() >= 2) {
         ObsFast a = {};
         for (int c = 0; c < 4; ++c) {
-            if (c & 1) observe_fast<WPH, NT, 1>(lds32, c, H, qp, a);
-            else observe_fast<WPH, NT, 0>(lds32, c, H, qp, a);
+            if (c & 1) observe_fast<WPH, NT, 1>(lds32, c, H, qp, a, tid);
+            else observe_fast<WPH, NT, 0>(lds32, c, H, qp, a, tid);
         }
         const int32_t nw = (int32_t)a.nw;
         v[OBS_M] = 2 * (int64_t)a.ps - 4 * (int64_t)nw;
@@ -796,12 +803,12 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     if (SHADOW) {
         sh_prop = wave_sum<unsigned long long>(sh_prop);
         sh_mis = wave_sum<unsigned long long>(sh_mis);
-        if ((tid & 63) == 0 && sh_prop) {
+        if ((tid & 63) == 0 && sh_prop && active) {
             atomicAdd(&st.shadow[2 * d.conf + 0], sh_prop);
             atomicAdd(&st.shadow[2 * d.conf + 1], sh_mis);
         }
     }
-    for (int i = tid; i < N / 32; i += NT) {            // bytes 4b -> bits
+    for (int i = tid; active && i < N / 32; i += NT) {  // bytes 4b -> bits
         const uint4 v0 = reinterpret_cast<const uint4 *>(lds32)[2 * i], v1 = reinterpret_cast<const uint4 *>(lds32)[2 * i + 1];
         const uint32_t b[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
         uint32_t W = 0;
@@ -809,8 +816,8 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         for (int k = 0; k < 8; ++k) W |= (((((b[k] >> 2) & 0x01010101u) * 0x01020408u) >> 24) & 0xFu) << (4 * k);
         reinterpret_cast<uint32_t *>(gsp)[i] = W;
     }
-    block_reduce_obs<NT>(v, red);
-    if (tid == 0) {
+    block_reduce_obs<NT>(v, red, tid);
+    if (tid == 0 && active) {
         v[OBS_EQ] -= 4 * cc.qsum;
         int64_t *o = st.obs + (int64_t)lat * OBS_N;
 #pragma unroll
@@ -856,17 +863,50 @@ void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
     resident_body<WPH, NT, SHADOW, HC>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + lds_words));
 }
 
+// A bundle: G lattices of size L = 8 WPH swept side by side by one block of
+// 1024 threads, a group of 1024 / G threads and a part of the LDS each (small
+// lattices keep few of 1024 threads busy: L = 16 has 128 words per layer pair
+// of a colour plane).  The bundle's lattices are lat .. lat + cnt - 1; groups
+// from cnt on repeat lattice lat without writing to global memory.
+template <int WPH, int G, int HC>
+__device__ __forceinline__ void bundle_body(const DeviceState &st, int lat, int cnt, uint32_t *lds32, int words)
+{
+    constexpr int NTG = 1024 / G;
+    static_assert(NTG % 64 == 0, "a group is whole waves");
+    // the group is wave-uniform (whole waves), so its lattice's descriptor
+    // and everything derived from it stay scalar
+    const int g = __builtin_amdgcn_readfirstlane((int)threadIdx.x / NTG), tid = threadIdx.x - g * NTG;
+    const bool active = g < cnt;
+    const int my = active ? lat + g : lat;
+    const LatDesc d = st.lats[my];
+    resident_body<WPH, NTG, false, HC>(st, d, my, lds32 + g * words,
+                                       reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + G * words), tid, active);
+}
+
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
-// CUBIC: every lattice of the launch has H = L.
+// CUBIC: every lattice of the launch has H = L.  With 1024 threads (FP32) a
+// work-list entry may be a bundle of up to res_bundle(L) lattices of one
+// config (entry = first lattice + (count - 1) << BUNDLE_SHIFT).
 template <int NT, bool SHADOW, bool CUBIC = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
 void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
 {
     extern __shared__ uint32_t lds32[];
-    const int lat = st.order[ord0 + blockIdx.x];
-    if (lat < 0) return;
+    const int e = st.order[ord0 + blockIdx.x];
+    if (e < 0) return;
+    const int lat = e & BUNDLE_LAT_MASK;
     const LatDesc d = st.lats[lat];
-    int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + (st.flips ? 2 : 1) * (d.N / 4));
+    const int words = (st.flips ? 2 : 1) * (d.N / 4);   // a lattice (and its flip counts) in LDS
+    if constexpr (NT == 1024 && !SHADOW) {
+        const int cnt = (e >> BUNDLE_SHIFT) + 1;
+        switch (d.L) {
+            case 16: bundle_body<2, res_bundle(16), CUBIC ? 16 : 0>(st, lat, cnt, lds32, words); return;
+            case 24: bundle_body<3, res_bundle(24), CUBIC ? 24 : 0>(st, lat, cnt, lds32, words); return;
+            case 32: bundle_body<4, res_bundle(32), CUBIC ? 32 : 0>(st, lat, cnt, lds32, words); return;
+            default: break;
+        }
+    }
+    int64_t (*red)[OBS_N] = reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + words);
     switch (d.L) {
         case 8: resident_body<1, NT, SHADOW, CUBIC ? 8 : 0>(st, d, lat, lds32, red); break;
         case 16: resident_body<2, NT, SHADOW, CUBIC ? 16 : 0>(st, d, lat, lds32, red); break;

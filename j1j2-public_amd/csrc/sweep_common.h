@@ -63,9 +63,12 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
-// Block-wide sum of OBS_N int64 values; result valid in thread 0.
+// Sum of OBS_N int64 values over a group of NT threads (the block, or one of
+// the groups a block is split into, NT a multiple of 64 then), tid the thread's
+// index in its group; the result is valid in the group's thread 0.  red holds
+// a row per wave of the block.
 template <int NT>
-__device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N])
+__device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N], int tid = threadIdx.x)
 {
     constexpr int NW = (NT + 63) / 64;      // waves, the last one possibly partial
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -75,10 +78,10 @@ __device__ void block_reduce_obs(int64_t v[OBS_N], int64_t (*red)[OBS_N])
 #pragma unroll
         for (int k = 0; k < OBS_N; ++k) red[wave][k] = v[k];
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (tid == 0)
         for (int w = 1; w < NW; ++w)
 #pragma unroll
-            for (int k = 0; k < OBS_N; ++k) v[k] += red[w][k];
+            for (int k = 0; k < OBS_N; ++k) v[k] += red[wave + w][k];
 }
 
 struct SweepCtx {
