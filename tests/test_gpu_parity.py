@@ -79,6 +79,27 @@ def test_resident_block_sizes_match_oracle(nt, monkeypatch):
     eng.close()
 
 
+@pytest.mark.parametrize("flags", [LOCNEM, LOCNEM | J.FLAG_WRITENUMFLIPS])
+def test_bundles_with_partial_groups_match_oracle(flags, monkeypatch):
+    """The 1024-thread launch over several sizes sweeps small lattices in
+    bundles (8 per block at L = 16, 2 at L = 24 and 32).  Replica counts that
+    are not multiples of the bundle size leave groups with no lattice of
+    their own (they repeat the first lattice's sweeps and write nothing):
+    every lattice, including the last of each config, and the local-nematic
+    sums and flip counts shared or kept per block, against the oracle."""
+    say(f"case bundles with partial groups, flags {flags}")
+    monkeypatch.setenv("J1J2_RES_THREADS", "1024")
+    # (L, replicas, configs); L = 40 (no bundle) keeps flip counts resident
+    sizes = [(16, 11, 2), (24, 3, 1), (32, 5, 1), (40, 2, 1)]
+    groups = []
+    for k, (L, R, C) in enumerate(sizes):
+        groups.append(J.make_params(L, L, nreplica=R, nconf=C, iconf0=1 + 4 * k, flags=flags, seed=SEED,
+                                    ptmeasindex=2))
+    eng, n = run_and_compare(groups, ["hot", "str"] * 3, 2, 4, flips=bool(flags & J.FLAG_WRITENUMFLIPS))
+    assert eng.sweep_path()[0] == 1 and n == sum(C for _, _, C in sizes)
+    assert "nt1024" in eng.launch_plan() and "bundled" in eng.launch_plan(), eng.launch_plan()
+
+
 @pytest.mark.parametrize("L,H,tmin,tmax", [(16, 8, 0.02, 0.05), (16, 8, 2.0e4, 1.0e5), (64, 4, 0.02, 0.05),
                                           (64, 4, 2.0e4, 1.0e5)])
 def test_fp32_extreme_temperatures_match_oracle(L, H, tmin, tmax):
