@@ -157,8 +157,9 @@ __device__ __forceinline__ BitSites bit_sites(const uint32_t *lds32, int c, int 
 // half, as f16_even / f16_odd of the byte form) and of the bond counts Z and
 // D.  The x neighbours and the diagonals come from 5-site windows of one dword
 // starting at the lower x (bit k of a window is the -x side of site k, bit
-// k + 1 its +x side), so each is one extract: a multiply by 0x00810204 puts
-// window bit k at bit 8k+2 and bit k+1 at bit 8k+3; a multiply by 0x108421
+// k + 1 its +x side), so each is one extract: a multiply by 0x00800200 puts
+// window bit k at bits 9 + k and 23 + k (the views' bits 10 and 26 then hold
+// window bits 1 and 3; shifted by one, bits 0 and 2 or 2 and 4); a multiply by 0x108421
 // puts window bit b at bit 6b (no two products overlap, so no carries), the
 // two windows' sum u has the per-bit counts 0..2 in 6-bit fields and u + (u <<
 // 6) the diagonal count D_k = u_k + u_{k+1} in field k+1.  Views of D: D_0
@@ -179,12 +180,14 @@ __device__ __forceinline__ BitView bit_view(const uint32_t *lds32, int c, int NU
     const int p1 = 2 * (c ^ 1) * NU + gp.h, p2 = 2 * (c ^ 2) * NU + gp.h, p3 = 2 * (c ^ 3) * NU + gp.h;
     BitView v;
     v.S = nib(lds32, 2 * (c * NU + n.u) + gp.h, pos0);
-    const uint32_t tx = __builtin_amdgcn_ubfe(lds32[p1 + 2 * n.u], (uint32_t)pmin, 5u) * 0x00810204u;
-    const uint32_t txp = tx >> 1;
-    v.xme = f16_even(tx & 0x04040404u);
+    // x window bit k at bits 9 + k and 23 + k: the f16 views take bits 10
+    // and 26 (sites 0 and 2 of an even view, 1 and 3 of an odd one), and the
+    // +x view of sites 0, 2 is the -x view of sites 1, 3 (window bits 1, 3)
+    const uint32_t tx = __builtin_amdgcn_ubfe(lds32[p1 + 2 * n.u], (uint32_t)pmin, 5u) * 0x00800200u;
     v.xmo = tx & 0x04000400u;
-    v.xpe = f16_even(txp & 0x04040404u);
-    v.xpo = txp & 0x04000400u;
+    v.xpe = v.xmo;
+    v.xme = (tx << 1) & 0x04000400u;
+    v.xpo = (tx >> 1) & 0x04000400u;
     const uint32_t yp = spread4(nib(lds32, p2 + 2 * n.uyp, pos0)), ym = spread4(nib(lds32, p2 + 2 * n.uym, pos0));
     v.ype = f16_even(yp);
     v.ypo = f16_odd(yp);
