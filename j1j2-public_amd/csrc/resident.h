@@ -885,8 +885,9 @@ __device__ __forceinline__ void bundle_body(const DeviceState &st, int lat, int 
 
 // Uneven batches: lattices of several sizes (L in {8, ..., 48}) in one launch.
 // CUBIC: every lattice of the launch has H = L.  With 1024 threads (FP32) a
-// work-list entry may be a bundle of up to res_bundle(L) lattices of one
-// config (entry = first lattice + (count - 1) << BUNDLE_SHIFT).
+// work-list entry may be a bundle of 2 .. res_bundle(L) lattices of one
+// config (entry = first lattice + (count - 1) << BUNDLE_SHIFT), which needs
+// res_bundle(L) lattices' LDS.
 template <int NT, bool SHADOW, bool CUBIC = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : 4, 8)))
 void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
@@ -898,8 +899,10 @@ void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
     const LatDesc d = st.lats[lat];
     const int words = (st.flips ? 2 : 1) * (d.N / 4);   // a lattice (and its flip counts) in LDS
     if constexpr (NT == 1024 && !SHADOW) {
+        // a bundle of one lattice takes the single-lattice path below (its
+        // LDS is one lattice's, as for any list without bundles)
         const int cnt = (e >> BUNDLE_SHIFT) + 1;
-        switch (d.L) {
+        if (cnt > 1) switch (d.L) {
             case 16: bundle_body<2, res_bundle(16), CUBIC ? 16 : 0>(st, lat, cnt, lds32, words); return;
             case 24: bundle_body<3, res_bundle(24), CUBIC ? 24 : 0>(st, lat, cnt, lds32, words); return;
             case 32: bundle_body<4, res_bundle(32), CUBIC ? 32 : 0>(st, lat, cnt, lds32, words); return;
