@@ -183,7 +183,10 @@ __device__ __forceinline__ BitView bit_view(const uint32_t *lds32, int c, int NU
     // x window bit k at bits 9 + k and 23 + k: the f16 views take bits 10
     // and 26 (sites 0 and 2 of an even view, 1 and 3 of an odd one), and the
     // +x view of sites 0, 2 is the -x view of sites 1, 3 (window bits 1, 3)
-    const uint32_t tx = __builtin_amdgcn_ubfe(lds32[p1 + 2 * n.u], (uint32_t)pmin, 5u) * 0x00800200u;
+    uint32_t tx = __builtin_amdgcn_ubfe(lds32[p1 + 2 * n.u], (uint32_t)pmin, 5u) * 0x00800200u;
+    // (hidden from the optimiser, which would otherwise fold tx << 1 into a
+    // second multiply, by a constant too wide for the 24-bit multiplier)
+    asm volatile("" : "+v"(tx));
     v.xmo = tx & 0x04000400u;
     v.xpe = v.xmo;
     v.xme = (tx << 1) & 0x04000400u;
