@@ -339,10 +339,13 @@ __device__ __forceinline__ void bit_group(uint32_t *lds32, const SweepCtx &cx, c
     constexpr int G0 = Gm::G0, N0 = Gm::N0, N1 = Gm::N1;
     uint32_t *uw = lds32 + 2 * (c * NU + n.u);
     atomicXor(uw + gp.h, f << gp.pos);
-    // (masks fixed per thread, so one conditional LDS op and no branch tree)
-    const uint32_t fm = (j == 0 || j == G0) ? 1u : 0u, lm = (j == G0 - 1 || j == G - 1) ? 1u : 0u;
-    const uint32_t fsh = j == 0 ? N1 + 1 : N0 + 1;
-    const uint32_t cm = ((f & fm) << fsh) | ((f >> 3) & lm);
+    // (one bit-field extract and one shift with operands fixed per thread --
+    // a first group's bit 0 to its copy's place, a last group's bit 3 to bit 0,
+    // a width of 0 for the others -- so one conditional LDS op and no branch tree)
+    const bool first = j == 0 || j == G0, last = j == G0 - 1 || j == G - 1;
+    const uint32_t cb = last ? 3u : 0u, cw = (first || last) ? 1u : 0u;
+    const uint32_t csh = first ? (j == 0 ? N1 + 1 : N0 + 1) : 0u;
+    const uint32_t cm = __builtin_amdgcn_ubfe(f, cb, cw) << csh;
     if (cm) atomicXor(uw + (1 - gp.h), cm);
 }
 
