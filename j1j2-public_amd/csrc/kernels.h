@@ -117,10 +117,11 @@ size_t bits_slab_x_words(int32_t nlat);     // slab_x words for nlat lattices (L
 bool resident_nt_valid(int32_t L, int32_t nt);
 // The resident kernel with NT threads per block for lattices of size L (0: a
 // launch of several sizes), FP64 shadow or not, H = L for every lattice
-// (cubic) or not; defined in kernels_res<NT>.hip.
+// (cubic) or not, for an engine that counts flips or not; defined in
+// kernels_res<NT>.hip.
 using ResKernel = void (*)(DeviceState, int32_t, int32_t);
 template <int NT>
-ResKernel resident_kernels(int32_t L, bool shadow, bool cubic);
+ResKernel resident_kernels(int32_t L, bool shadow, bool cubic, bool flips);
 size_t resident_lds_limit();
 // Raise the dynamic-LDS limit of the resident kernel (once, before any launch or capture).
 hipError_t setup_resident(size_t max_lds);

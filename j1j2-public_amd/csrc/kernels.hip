@@ -475,13 +475,13 @@ size_t resident_lds_limit()
     return lim > RES_STATIC_LDS ? lim - RES_STATIC_LDS : 0;
 }
 
-ResKernel pick_resident(int32_t L, int32_t nt, bool shadow, bool cubic = false)
+ResKernel pick_resident(int32_t L, int32_t nt, bool shadow, bool cubic = false, bool flips = false)
 {
     switch (nt) {
-        case 256: return resident_kernels<256>(L, shadow, cubic);
-        case 512: return resident_kernels<512>(L, shadow, cubic);
-        case 576: return resident_kernels<576>(L, shadow, cubic);
-        case 1024: return resident_kernels<1024>(L, shadow, cubic);
+        case 256: return resident_kernels<256>(L, shadow, cubic, flips);
+        case 512: return resident_kernels<512>(L, shadow, cubic, flips);
+        case 576: return resident_kernels<576>(L, shadow, cubic, flips);
+        case 1024: return resident_kernels<1024>(L, shadow, cubic, flips);
         default: return nullptr;
     }
 }
@@ -491,8 +491,9 @@ hipError_t setup_resident(size_t max_lds)
     for (int L : {8, 16, 24, 32, 40, 48, 0})
         for (int nt : {256, 512, 576, 1024})
             for (bool sh : {false, true})
-                for (bool cubic : {false, true}) {
-                    const ResKernel k = pick_resident(L, nt, sh, cubic);
+                for (int v = 0; v < 4; ++v) {
+                    const bool cubic = v & 1, flips = v >> 1;
+                    const ResKernel k = pick_resident(L, nt, sh, cubic, flips);
                     if (!k) continue;
                     hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                        (int)(max_lds + RES_STATIC_LDS));
@@ -509,9 +510,12 @@ hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t no
                                  int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s)
 {
     if (nord <= 0) return hipSuccess;
-    const ResKernel k = pick_resident(L, nt, shadow, L > 0 ? H == L : H < 0);
+    const bool cubic = L > 0 ? H == L : H < 0, flips = st.flips != nullptr;
+    const ResKernel k = pick_resident(L, nt, shadow, cubic, flips);
     if (!k) return hipErrorInvalidValue;
-    const size_t lattice = (size_t)max_n * (st.flips ? 2 : 1);      // with flip counts: a byte per site more
+    // with flip counts: a byte per site more, unless the kernel keeps them in registers
+    const bool lds_counts = flips && k == pick_resident(L, nt, shadow, cubic, false);
+    const size_t lattice = (size_t)max_n * (lds_counts ? 2 : 1);
     hipLaunchKernelGGL(k, dim3(nord), dim3(nt), lattice + RES_STATIC_LDS, s, st, ord0, nord);
     return hipGetLastError();
 }

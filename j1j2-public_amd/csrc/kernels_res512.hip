@@ -5,9 +5,9 @@
 namespace j1j2 {
 
 template <>
-ResKernel resident_kernels<512>(int32_t L, bool shadow, bool cubic)
+ResKernel resident_kernels<512>(int32_t L, bool shadow, bool cubic, bool flips)
 {
-    return shadow ? resident_kernel<512, true>(L, cubic) : resident_kernel<512, false>(L, cubic);
+    return shadow ? resident_kernel<512, true>(L, cubic, flips) : resident_kernel<512, false>(L, cubic, flips);
 }
 
 }  // namespace j1j2

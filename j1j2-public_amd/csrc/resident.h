@@ -285,6 +285,14 @@ constexpr bool unrolled_pairs()
     return HC > 0 && ZP >= 1 && (HC / 2 + ZP - 1) / ZP <= 6;
 }
 
+// Steps of an unrolled colour pass (two words each).
+template <int WPH, int NT, int HC>
+constexpr int pair_steps()
+{
+    constexpr int ZP = pair_zs<WPH, NT>();
+    return unrolled_pairs<WPH, NT, HC>() ? (HC / 2 + ZP - 1) / ZP : 0;
+}
+
 template <int WPH, int NT, int HX>
 __device__ __forceinline__ void observe_fast(const uint32_t *lds32, int c, int H, const QPlanes &qp,
                                              ObsFast &a, int tid)
@@ -402,10 +410,11 @@ __device__ __forceinline__ void x_views(uint32_t A, uint32_t B, uint32_t &xpe, u
     }
 }
 
-template <int WPH, int HX, bool SHADOW, int ODD>
+template <int WPH, int HX, bool SHADOW, int ODD, bool RC = false>
 __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int own,
                                            uint32_t m, int w, const NbPtr &P, const U4 &hi,
-                                           unsigned long long &sh_prop, unsigned long long &sh_mis)
+                                           unsigned long long &sh_prop, unsigned long long &sh_mis,
+                                           uint32_t *fa = nullptr)
 {
     constexpr int HALF = 4 * WPH;
     const uint32_t S = *P.own;
@@ -499,8 +508,10 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
             if (((open >> j) & 1u) && decide_f32_open(ys[j], site_word(ha[j], la[j], ODD))) flip |= 4u << (8 * j);
     }
     *P.own = S ^ flip;
-    // flip counts (WRITENUMFLIPS): this thread alone touches these four sites' counters in this pass
-    if (cx.cnt_off) P.own[cx.cnt_off] += flip >> 2;
+    // flip counts (WRITENUMFLIPS): this thread alone touches these four sites'
+    // counters in this pass, in a register (RC) or in the LDS after the lattice
+    if constexpr (RC) *fa += flip >> 2;
+    else if (cx.cnt_off) P.own[cx.cnt_off] += flip >> 2;
 }
 
 // The odd layer of a pair from the even layer's addresses P (layer 2 zp, y
@@ -529,9 +540,10 @@ __device__ __forceinline__ NbPtr odd_layer(const NbPtr &P, const uint32_t *yf, c
 // One colour pass over plane C.  HC > 0: the lattice height is HC, known at
 // compile time.  Work comes in layer pairs (2 zp, 2 zp + 1), one Philox call
 // per pair of words (m = zp PER_Z + the word's index within its layer).
-template <int WPH, int NT, int C, bool SHADOW, int HC = 0>
+template <int WPH, int NT, int C, bool SHADOW, int HC = 0, bool RC = false>
 __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int H,
-                                            unsigned long long &sh_prop, unsigned long long &sh_mis, int tid)
+                                            unsigned long long &sh_prop, unsigned long long &sh_mis, int tid,
+                                            uint32_t *fa = nullptr)
 {
     constexpr int HX = C & 1, PY = C >> 1;
     constexpr int HALF = 4 * WPH, PER_Z = HALF * WPH;    // dwords per z-layer of a plane
@@ -569,11 +581,12 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
             const U4 hi = philox4x32_10_u(m, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
             NbPtr Q = nb_shift(P, kk * STEP);
             if (zp == 0) Q.zm += H * PER_Z;                // periodic in z
-            update_word<WPH, HX, SHADOW, 0>(lds32, cx, cc, own0 + kk * STEP, m, w, Q, hi, sh_prop, sh_mis);
+            update_word<WPH, HX, SHADOW, 0, RC>(lds32, cx, cc, own0 + kk * STEP, m, w, Q, hi, sh_prop, sh_mis,
+                                                RC ? fa + 2 * kk : nullptr);
             NbPtr Q1 = nb_shift(odd_layer<PY, PER_Z>(P, yf, dfa, dfb), kk * STEP);
             if (zp == npairs - 1) Q1.zp -= H * PER_Z;
-            update_word<WPH, HX, SHADOW, 1>(lds32, cx, cc, own0 + kk * STEP + PER_Z, m, w, Q1, hi, sh_prop,
-                                            sh_mis);
+            update_word<WPH, HX, SHADOW, 1, RC>(lds32, cx, cc, own0 + kk * STEP + PER_Z, m, w, Q1, hi, sh_prop,
+                                                sh_mis, RC ? fa + 2 * kk + 1 : nullptr);
         };
         if constexpr (unrolled_pairs<WPH, NT, HC>()) {
             // a thread (first pair zp0 < ZP) takes NK = ceil(HC/2 / ZP) steps,
@@ -587,6 +600,7 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
                 step(kk, zp);
             }
         } else {
+            static_assert(!RC, "register flip counts need the unrolled pass");
             const int nk = (npairs - zp0 + ZP - 1) / ZP;
             for (int k = 0; k < nk; k += TRIP) {
 #pragma unroll
@@ -595,6 +609,7 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
             }
         }
     } else {
+        static_assert(!RC, "register flip counts need the unrolled pass");
         for (int mm = tid; mm < nq / 2; mm += NT) {
             const int zp = mm / PER_Z, tz = mm - zp * PER_Z;
             const U4 hi = philox4x32_10_u((uint32_t)mm, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
@@ -610,12 +625,44 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
     }
 }
 
+// confnumflips(slot, site) += the counts a thread kept in registers through
+// the unrolled colour passes (word kk, layer o of plane C: the pass's own0 +
+// kk STEP + o PER_Z); a plane's counters are all requested before the first
+// is added to.
+template <int WPH, int NT, int HC, int RCW>
+__device__ __forceinline__ void flush_reg_counts(int32_t *flips, const uint32_t (&fa)[4][RCW], int tid)
+{
+    constexpr int PER_Z = 4 * WPH * WPH, ZP = pair_zs<WPH, NT>(), STEP = 2 * ZP * PER_Z, NK = RCW / 2;
+    constexpr int nq = PER_Z * HC;
+    if (tid >= ZP * PER_Z) return;
+    const int tz = tid % PER_Z, zp0 = tid / PER_Z;
+    int4 *fl = reinterpret_cast<int4 *>(flips);
+#pragma unroll
+    for (int C = 0; C < 4; ++C) {
+        const int own0 = C * nq + 2 * zp0 * PER_Z + tz;
+        int4 g[RCW];
+#pragma unroll
+        for (int q = 0; q < RCW; ++q)
+            if (zp0 + (q >> 1) * ZP < HC / 2 && fa[C][q]) g[q] = fl[own0 + (q >> 1) * STEP + (q & 1) * PER_Z];
+#pragma unroll
+        for (int q = 0; q < RCW; ++q) {
+            if (zp0 + (q >> 1) * ZP >= HC / 2 || !fa[C][q]) continue;
+            const uint32_t n4 = fa[C][q];
+            g[q].x += (int)(n4 & 0xFFu);
+            g[q].y += (int)((n4 >> 8) & 0xFFu);
+            g[q].z += (int)((n4 >> 16) & 0xFFu);
+            g[q].w += (int)(n4 >> 24);
+            fl[own0 + (q >> 1) * STEP + (q & 1) * PER_Z] = g[q];
+        }
+    }
+}
+
 // The sweeps of one lattice by a group of NT threads (tid: index in the
 // group; the whole block, or one of several groups that sweep a lattice each
 // in their own part of the LDS).  active = false: the group repeats the work
 // of another group's lattice in its own LDS (a bundle with fewer lattices
 // than groups) and writes nothing to global memory; it takes every barrier.
-template <int WPH, int NT, bool SHADOW, int HC = 0>
+template <int WPH, int NT, bool SHADOW, int HC = 0, bool RC = false>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
                                               uint32_t *lds32, int64_t (*red)[OBS_N], int tid = threadIdx.x,
                                               bool active = true)
@@ -706,8 +753,10 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     // flip counts of this launch, one byte per site right after the lattice
     // (at most PTSWAPEVER <= 255 per site), added to the slot's counters at the end
     const bool counting = st.flips != nullptr;
-    if (counting)
+    if (counting && !RC)
         for (int i = tid; i < N / 4; i += NT) lds32[N / 4 + i] = 0u;
+    // RC: the counts of the words a thread updates stay in its registers
+    uint32_t fa[4][RC ? 2 * pair_steps<WPH, NT, HC>() : 1] = {};
 
     const ConfDesc cd = st.confs[d.conf];
     const ConfConst cc = st.cconst[d.conf];
@@ -725,7 +774,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
-    cx.cnt_off = counting ? N / 4 : 0;
+    cx.cnt_off = counting && !RC ? N / 4 : 0;
     const uint64_t S0 = clock_cycle(st) * (uint64_t)st.ptswapever;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
@@ -738,15 +787,17 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
             switch (c) {
-                case 0: colour_pass<WPH, NT, 0, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
-                case 1: colour_pass<WPH, NT, 1, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
-                case 2: colour_pass<WPH, NT, 2, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
-                default: colour_pass<WPH, NT, 3, SHADOW, HC>(lds32, cx, cc, H, sh_prop, sh_mis, tid); break;
+                case 0: colour_pass<WPH, NT, 0, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[0]); break;
+                case 1: colour_pass<WPH, NT, 1, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[1]); break;
+                case 2: colour_pass<WPH, NT, 2, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[2]); break;
+                default: colour_pass<WPH, NT, 3, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[3]); break;
             }
             __syncthreads();
         }
     }
-    if (counting && active) {
+    if constexpr (RC) {
+        if (counting && active) flush_reg_counts<WPH, NT, HC>(st.flips + cd.flip_off + (int64_t)slot * N, fa, tid);
+    } else if (counting && active) {
         // confnumflips(slot, site) += this launch's counts (internal site order)
         // (in batches of four words, the counters of a batch all requested
         // before the first is added to)
@@ -847,9 +898,20 @@ constexpr int res_waves()
     return 4;
 }
 
+// With flip counts, the 256-thread blocks of the unrolled passes keep each
+// thread's counts in registers (8 per step and colour pass, at most 32):
+// those blocks run at four waves per SIMD with room for them, and the LDS
+// then holds the lattice alone, so as many blocks fit a CU as without counts.
+template <int WPH, int NT, int HC>
+constexpr bool reg_counts()
+{
+    return NT == 256 && unrolled_pairs<WPH, NT, HC>() && 8 * pair_steps<WPH, NT, HC>() <= 32 &&
+           res_waves<WPH, NT, HC>() == 4;
+}
+
 // One geometry.  HC > 0: every lattice of the launch has height HC (cubic
-// lattices, H = L).
-template <int WPH, int NT, bool SHADOW, int HC = 0>
+// lattices, H = L).  RC: flip counts in registers (reg_counts), none in the LDS.
+template <int WPH, int NT, bool SHADOW, int HC = 0, bool RC = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : res_waves<WPH, NT, HC>(), 8)))
 void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
 {
@@ -859,8 +921,9 @@ void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
     const int lat = st.order[ord0 + blockIdx.x];
     if (lat < 0) return;
     const LatDesc d = st.lats[lat];
-    const int lds_words = (st.flips ? 2 : 1) * (d.N / 4);     // lattice (and flip counts)
-    resident_body<WPH, NT, SHADOW, HC>(st, d, lat, lds32, reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + lds_words));
+    const int lds_words = (st.flips && !RC ? 2 : 1) * (d.N / 4);     // lattice (and flip counts)
+    resident_body<WPH, NT, SHADOW, HC, RC>(st, d, lat, lds32,
+                                           reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + lds_words));
 }
 
 // A bundle: G lattices of size L = 8 WPH swept side by side by one block of
@@ -924,20 +987,30 @@ void k_sweep_resident_mixed(DeviceState st, int32_t ord0, int32_t nord)
 // reduction scratch after the lattice in the resident sweep's dynamic LDS
 constexpr size_t RES_STATIC_LDS = sizeof(int64_t) * (1024 / 64) * OBS_N;
 
+// The FP32 kernel of cubic lattices of size 8 WPH; with flip counts the one
+// that keeps them in registers where there is one.
+template <int WPH, int NT>
+ResKernel cubic_kernel(bool flips)
+{
+    if constexpr (reg_counts<WPH, NT, 8 * WPH>())
+        if (flips) return k_sweep_resident<WPH, NT, false, 8 * WPH, true>;
+    return k_sweep_resident<WPH, NT, false, 8 * WPH>;
+}
+
 // The kernel for lattices of size L (0: a launch of several sizes).  cubic:
 // every lattice of the launch has H = L (the FP32 kernels then take the
-// height as a compile-time constant).
+// height as a compile-time constant).  flips: the engine counts flips.
 template <int NT, bool SHADOW>
-ResKernel resident_kernel(int32_t L, bool cubic)
+ResKernel resident_kernel(int32_t L, bool cubic, bool flips)
 {
     if (cubic && !SHADOW) {
         switch (L) {
-            case 8: return k_sweep_resident<1, NT, false, 8>;
-            case 16: return k_sweep_resident<2, NT, false, 16>;
-            case 24: return k_sweep_resident<3, NT, false, 24>;
-            case 32: return k_sweep_resident<4, NT, false, 32>;
-            case 40: return k_sweep_resident<5, NT, false, 40>;
-            case 48: return k_sweep_resident<6, NT, false, 48>;
+            case 8: return cubic_kernel<1, NT>(flips);
+            case 16: return cubic_kernel<2, NT>(flips);
+            case 24: return cubic_kernel<3, NT>(flips);
+            case 32: return cubic_kernel<4, NT>(flips);
+            case 40: return cubic_kernel<5, NT>(flips);
+            case 48: return cubic_kernel<6, NT>(flips);
             default: return k_sweep_resident_mixed<NT, false, true>;
         }
     }

@@ -247,6 +247,20 @@ def test_flip_counts_match_oracle(flags, path):
     assert eng.sweep_path()[0] == path
 
 
+@pytest.mark.parametrize("nt", [256, 512])
+def test_flip_counts_l32_match_oracle(monkeypatch, nt):
+    """Flip counts of cubic L = 32 lattices: in registers through the colour
+    passes (256-thread blocks) or in the LDS beside the lattice (512), added
+    to the slot's counters at the end of each of 20 launches, with the
+    local-nematic sums."""
+    say(f"case WRITENUMFLIPS L=32 nt {nt}")
+    monkeypatch.setenv("J1J2_RES_THREADS", str(nt))
+    g = J.make_params(32, 32, nreplica=3, nconf=2, iconf0=3, seed=SEED, ptmeasindex=2,
+                      flags=J.FLAG_WRITENUMFLIPS | LOCNEM)
+    eng, n = run_and_compare([g], ["hot", "str"], 8, 12, flips=True)
+    assert n == 2 and eng.sweep_path()[0] == 1 and f"L32 nt{nt}" in eng.launch_plan(), eng.launch_plan()
+
+
 def test_flip_counts_uneven_batch_match_oracle():
     """Flip counts over an uneven batch: L = 16, 24 and 40 on the LDS-resident
     sweep (lattice and counts fit one CU's LDS), L = 48 on the global-memory
