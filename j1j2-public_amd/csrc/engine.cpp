@@ -311,7 +311,10 @@ int reset_config(Engine &E, int conf)
         HIPCHK(hipMemsetAsync(E.st.att + cd.pair_base, 0, sizeof(int64_t) * (nrep - 1), E.stream));
         HIPCHK(hipMemsetAsync(E.st.acc + cd.pair_base, 0, sizeof(int64_t) * (nrep - 1), E.stream));
     }
-    if (E.st.flips) HIPCHK(hipMemsetAsync(E.st.flips + cd.flip_off, 0, sizeof(int32_t) * nrep * cd.N, E.stream));
+    if (E.st.flips) {
+        HIPCHK(hipMemsetAsync(E.st.flips + cd.flip_off, 0, sizeof(int32_t) * nrep * cd.N, E.stream));
+        HIPCHK(hipMemsetAsync(E.st.flips_pend + cd.flip_off, 0, (size_t)nrep * cd.N, E.stream));
+    }
     HIPCHK(hipMemsetAsync(E.st.shadow + 2 * conf, 0, 2 * sizeof(unsigned long long), E.stream));
     return reset_measurement(E, conf);
 }
@@ -1215,7 +1218,8 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         (rc = E->alloc(st.locnem_pending, nconf)))
         return rc;
     st.flips = nullptr;
-    if (flips && (rc = E->alloc(st.flips, E->nflips))) return rc;
+    st.flips_pend = nullptr;
+    if (flips && ((rc = E->alloc(st.flips, E->nflips)) || (rc = E->alloc(st.flips_pend, E->nflips)))) return rc;
     st.slab_x = st.slab_obs = nullptr;
     st.slab_err = nullptr;
     st.slab_spin_max = 1 << 21;                        // about seconds of polling
@@ -1686,8 +1690,12 @@ int j1j2_get_flips(void *handle, int32_t conf, int32_t *flips)
     const ConfDesc &cd = E->confs[conf];
     HIPCHK(hipSetDevice(E->device));
     std::vector<int32_t> buf;
-    if ((rc = d2h(*E, buf, E->st.flips + cd.flip_off, (int64_t)cd.nrep * cd.N))) return rc;
+    std::vector<uint8_t> pend;
+    const int64_t n = (int64_t)cd.nrep * cd.N;
+    if ((rc = d2h(*E, buf, E->st.flips + cd.flip_off, n)) || (rc = d2h(*E, pend, E->st.flips_pend + cd.flip_off, n)))
+        return rc;
     HIPCHK(hipStreamSynchronize(E->stream));
+    for (int64_t i = 0; i < n; ++i) buf[i] += pend[i];
     for (int s = 0; s < cd.nrep; ++s)
         from_internal<int32_t, int32_t>(cd.L, cd.H, buf.data() + (int64_t)s * cd.N, flips + (int64_t)s * cd.N,
                                         [](int32_t v) { return v; });

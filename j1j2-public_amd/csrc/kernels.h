@@ -61,6 +61,8 @@ struct DeviceState {
     int64_t *series_len;       // nconf
     int64_t series_cap;
     int32_t *flips;            // per slot per site (optional)
+    uint8_t *flips_pend;       // per slot per site, beside flips: counts of the register-count sweep not
+                               // yet added to flips (each < 128; the total is flips + flips_pend)
     unsigned long long *shadow;// nconf x 2
     // bit-layout lattices split into z-slabs (one block each): per lattice
     // BIT_SLAB_MAX slots of exchange words -- 4 planes x 2 boundary layers x
@@ -84,7 +86,9 @@ struct DeviceState {
 // otherwise).  The result depends neither on nt nor on the order.  With flip
 // counts (st.flips set) each block keeps a byte per site after its lattice
 // in LDS (2 max_n bytes; PTSWAPEVER <= 255) and adds them to the slot's
-// counters at the end of the launch.
+// counters at the end of the launch; the kernels that keep them in registers
+// (256-thread blocks of cubic L = 32, PTSWAPEVER <= RC_MAX_SWAPEVER) need
+// the lattice's LDS alone and add them to st.flips_pend.
 hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t nord,
                                  int32_t max_n, int32_t L, int32_t H, int32_t nt, bool shadow, hipStream_t s);
 bool resident_supported(int32_t L);
@@ -96,6 +100,9 @@ bool resident_supported(int32_t L);
 constexpr int BUNDLE_SHIFT = 24;
 constexpr int32_t BUNDLE_LAT_MASK = (1 << BUNDLE_SHIFT) - 1;
 constexpr int res_bundle(int32_t L) { return L == 16 ? 8 : (L == 24 || L == 32) ? 2 : 1; }
+// The resident sweeps that keep flip counts in registers add them to a byte
+// per site (flips_pend), which must not carry: PTSWAPEVER at most this.
+constexpr int32_t RC_MAX_SWAPEVER = 127;
 // Bytes of LDS the resident kernels add after the lattice (reduction scratch).
 constexpr size_t RESIDENT_SCRATCH_LDS = sizeof(int64_t) * (1024 / 64) * 8;
 // Bit-layout resident sweep (kernels_bits.hip): L % 8 == 0, 48 < L <= 96, one

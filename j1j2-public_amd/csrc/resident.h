@@ -625,34 +625,49 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
     }
 }
 
-// confnumflips(slot, site) += the counts a thread kept in registers through
-// the unrolled colour passes (word kk, layer o of plane C: the pass's own0 +
-// kk STEP + o PER_Z); a plane's counters are all requested before the first
-// is added to.
+// The counts a thread kept in registers through the unrolled colour passes
+// (word kk, layer o of plane C: the pass's own0 + kk STEP + o PER_Z) go to
+// the slot's pending byte per site (four sites a word: bytes < 128 plus at
+// most PTSWAPEVER <= 127, no carries); a word whose bytes reach 128 is added
+// to confnumflips(slot, site) and cleared.  So a launch moves 8 bytes per four
+// sites, not the 32 of the 4-byte counters, and the counters once in about
+// 128 / PTSWAPEVER launches.  All of a thread's pending words are requested
+// before the first is added to (one memory round trip).
 template <int WPH, int NT, int HC, int RCW>
-__device__ __forceinline__ void flush_reg_counts(int32_t *flips, const uint32_t (&fa)[4][RCW], int tid)
+__device__ __forceinline__ void flush_reg_counts(int32_t *flips, uint8_t *pend8, const uint32_t (&fa)[4][RCW],
+                                                 int tid)
 {
-    constexpr int PER_Z = 4 * WPH * WPH, ZP = pair_zs<WPH, NT>(), STEP = 2 * ZP * PER_Z, NK = RCW / 2;
+    constexpr int PER_Z = 4 * WPH * WPH, ZP = pair_zs<WPH, NT>(), STEP = 2 * ZP * PER_Z;
     constexpr int nq = PER_Z * HC;
     if (tid >= ZP * PER_Z) return;
     const int tz = tid % PER_Z, zp0 = tid / PER_Z;
     int4 *fl = reinterpret_cast<int4 *>(flips);
+    uint32_t *pend = reinterpret_cast<uint32_t *>(pend8);
+    uint32_t p[4][RCW];
+#pragma unroll
+    for (int C = 0; C < 4; ++C)
+#pragma unroll
+        for (int q = 0; q < RCW; ++q)
+            if (zp0 + (q >> 1) * ZP < HC / 2 && fa[C][q])
+                p[C][q] = pend[C * nq + 2 * zp0 * PER_Z + tz + (q >> 1) * STEP + (q & 1) * PER_Z];
 #pragma unroll
     for (int C = 0; C < 4; ++C) {
         const int own0 = C * nq + 2 * zp0 * PER_Z + tz;
-        int4 g[RCW];
-#pragma unroll
-        for (int q = 0; q < RCW; ++q)
-            if (zp0 + (q >> 1) * ZP < HC / 2 && fa[C][q]) g[q] = fl[own0 + (q >> 1) * STEP + (q & 1) * PER_Z];
 #pragma unroll
         for (int q = 0; q < RCW; ++q) {
             if (zp0 + (q >> 1) * ZP >= HC / 2 || !fa[C][q]) continue;
-            const uint32_t n4 = fa[C][q];
-            g[q].x += (int)(n4 & 0xFFu);
-            g[q].y += (int)((n4 >> 8) & 0xFFu);
-            g[q].z += (int)((n4 >> 16) & 0xFFu);
-            g[q].w += (int)(n4 >> 24);
-            fl[own0 + (q >> 1) * STEP + (q & 1) * PER_Z] = g[q];
+            const int i = own0 + (q >> 1) * STEP + (q & 1) * PER_Z;
+            const uint32_t n4 = p[C][q] + fa[C][q];
+            if (n4 & 0x80808080u) {
+                int4 g = fl[i];
+                g.x += (int)(n4 & 0xFFu);
+                g.y += (int)((n4 >> 8) & 0xFFu);
+                g.z += (int)((n4 >> 16) & 0xFFu);
+                g.w += (int)(n4 >> 24);
+                fl[i] = g;
+                pend[i] = 0u;
+            } else
+                pend[i] = n4;
         }
     }
 }
@@ -796,7 +811,9 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         }
     }
     if constexpr (RC) {
-        if (counting && active) flush_reg_counts<WPH, NT, HC>(st.flips + cd.flip_off + (int64_t)slot * N, fa, tid);
+        if (counting && active)
+            flush_reg_counts<WPH, NT, HC>(st.flips + cd.flip_off + (int64_t)slot * N,
+                                          st.flips_pend + cd.flip_off + (int64_t)slot * N, fa, tid);
     } else if (counting && active) {
         // confnumflips(slot, site) += this launch's counts (internal site order)
         // (in batches of four words, the counters of a batch all requested

@@ -247,17 +247,19 @@ def test_flip_counts_match_oracle(flags, path):
     assert eng.sweep_path()[0] == path
 
 
-@pytest.mark.parametrize("nt", [256, 512])
-def test_flip_counts_l32_match_oracle(monkeypatch, nt):
+@pytest.mark.parametrize("nt,swapever,neq,nmess", [(256, 5, 8, 12), (256, 29, 4, 8), (256, 127, 2, 3),
+                                                     (256, 128, 1, 2), (512, 5, 8, 12)])
+def test_flip_counts_l32_match_oracle(monkeypatch, nt, swapever, neq, nmess):
     """Flip counts of cubic L = 32 lattices: in registers through the colour
-    passes (256-thread blocks) or in the LDS beside the lattice (512), added
-    to the slot's counters at the end of each of 20 launches, with the
-    local-nematic sums."""
-    say(f"case WRITENUMFLIPS L=32 nt {nt}")
+    passes (256-thread blocks, PTSWAPEVER <= 127), kept in a pending byte per
+    site and added to the slot's counters when a byte reaches 128 (several
+    times over the run at PTSWAPEVER = 29 and 127), or in the LDS beside the
+    lattice (PTSWAPEVER = 128, 512 threads), with the local-nematic sums."""
+    say(f"case WRITENUMFLIPS L=32 nt {nt} PTSWAPEVER {swapever}")
     monkeypatch.setenv("J1J2_RES_THREADS", str(nt))
-    g = J.make_params(32, 32, nreplica=3, nconf=2, iconf0=3, seed=SEED, ptmeasindex=2,
+    g = J.make_params(32, 32, nreplica=3, nconf=2, iconf0=3, seed=SEED, ptmeasindex=2, ptswapever=swapever,
                       flags=J.FLAG_WRITENUMFLIPS | LOCNEM)
-    eng, n = run_and_compare([g], ["hot", "str"], 8, 12, flips=True)
+    eng, n = run_and_compare([g], ["hot", "str"], neq, nmess, flips=True)
     assert n == 2 and eng.sweep_path()[0] == 1 and f"L32 nt{nt}" in eng.launch_plan(), eng.launch_plan()
 
 
