@@ -281,9 +281,10 @@ WORKLOADS = {
 }
 
 
-def other_workload(J, name, steps, warmup, with_fp64=True, with_cpu=True):
+def other_workload(J, name, steps, warmup, with_fp64=True, with_cpu=True, with_flips=False):
     desc, groups = WORKLOADS[name]
     fp32 = time_groups(J, groups, J.FP32_BATCHED, steps, warmup)
+    fl = time_groups(J, groups, J.FP32_BATCHED, steps, warmup, J.FLAG_WRITENUMFLIPS) if with_flips else None
     n64 = max(steps // 10, FP64_MIN_STEPS)
     fp64 = time_groups(J, groups, J.FP64_PLAIN, n64, max(warmup // 10, 2)) if with_fp64 else None
     su_launch = fp32["su"] / max(fp32["launches"], 1)
@@ -301,6 +302,8 @@ def other_workload(J, name, steps, warmup, with_fp64=True, with_cpu=True):
         "fp64_plain": ({"value": fp64["rate"], "steps": n64, "fp32_over_fp64": fp32["rate"] / fp64["rate"]}
                        if fp64 else None),
         "flip_mismatch_rate": {"fp32_vs_fp64": rate, "proposals": n},
+        "with_flip_counts": ({"value": fl["rate"], "sweep_path": J.PATH_NAMES[fl["path"]], "launch_plan": fl["plan"],
+                              "over_value": fl["rate"] / fp32["rate"]} if fl else None),
         # L = 16, one config: the reference runs it on one core (configs are
         # its unit of parallelism), 1000 cycles (~10 s)
         "cpu_baseline": (reference_rate("j1j2_cpu_l16", 16, 16, 1000, 1)
@@ -367,7 +370,8 @@ def main():
         copy_probe(args.copy_probe)
 
     if args.workload != "c3":
-        out = other_workload(J, args.workload, args.steps, args.warmup, not args.no_fp64, not args.no_cpu)
+        out = other_workload(J, args.workload, args.steps, args.warmup, not args.no_fp64, not args.no_cpu,
+                             args.flips)
         cpu = out["cpu_baseline"]
         if cpu and out["fp64_plain"]:
             out["fp64_plain"]["over_cpu_reference"] = out["fp64_plain"]["value"] / cpu["value"]

@@ -106,6 +106,7 @@ struct Engine {
     int32_t path = 0;                          // 0 FP64 plain, 1 FP32 resident, 2 FP32 global, 3 resident incl. bits
     bool all_resident = false, shadow = false, locnem = false, use_graph = true, timing = false;
     bool all_byte = false;                     // every group on the byte-layout resident kernel
+    bool all_one = false;                      // ... and one launch over all lattices fits the LDS
     int32_t res_nt = 256;                      // threads per block of the resident sweep (default plan)
     int32_t bit_slabs = 1;                     // z-slabs of the bit-layout lattices (default plan)
     int32_t uniform_L = 0;                     // common L of all groups, 0 if mixed
@@ -562,7 +563,7 @@ ResLaunch all_launch(const Engine &E, int32_t nt)
 Plan plan_from_choice(const Engine &E, const TuneChoice &t)
 {
     Plan P;
-    if (t.all_nt > 0 && E.all_byte) {
+    if (t.all_nt > 0 && E.all_one) {
         P.launches.push_back(all_launch(E, t.all_nt));
         return P;
     }
@@ -584,7 +585,7 @@ Plan plan_from_choice(const Engine &E, const TuneChoice &t)
 Plan default_plan(const Engine &E)
 {
     Plan P;
-    if (E.all_byte) {
+    if (E.all_one) {
         P.launches.push_back(all_launch(E, E.res_nt));
         return P;
     }
@@ -749,7 +750,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
     static constexpr int32_t ALL_NT[] = {256, 512, 576, 1024};     // one launch over all lattices
     constexpr int n_all = (int)(sizeof(ALL_NT) / sizeof(ALL_NT[0]));
     // cycles the candidates below may take (an upper bound: the call must hold them all)
-    const int need = TUNE_CYCLES * (TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_byte ? n_all : 0)) : 0));
+    const int need = TUNE_CYCLES * (TUNE_REPS * (int)maxc + (multi ? TUNE_REPS * (1 + (E.all_one ? n_all : 0)) : 0));
     if (ncycles - done < need) return 0;             // too short a call; tune on a later one
     int rc;
     const bool log = std::getenv("J1J2_TUNE_LOG") != nullptr;   // each timed candidate to stderr
@@ -806,7 +807,7 @@ int tune(Engine &E, int measure, int ncycles, int &done)
             alt_choice.push_back(per_choice);
             alt_choice.back().concurrent = true;
         }
-        if (E.all_byte)
+        if (E.all_one)
             for (int32_t nt : ALL_NT) {
                 Plan one;
                 one.launches.push_back(all_launch(E, nt));
@@ -1116,7 +1117,10 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         // byte layout (counts per launch <= PTSWAPEVER); the bit layout has no
         // room for them, so such lattices take the global-memory sweep
         const bool may = p.precision == J1J2_FP32_BATCHED && !(p.flags & J1J2_FLAG_FORCE_GLOBAL);
-        const size_t res_need = (size_t)G.N * (flips ? 2 : 1);
+        // (cubic L = 48 keeps its counts in the sites' own bytes instead)
+        const bool own_counts = byte_counts(p.L) && fast_counts_ok(p.L, p.H == p.L,
+                                                                   (p.flags & J1J2_FLAG_SHADOW_FP64) != 0, p.ptswapever);
+        const size_t res_need = (size_t)G.N * (flips && !own_counts ? 2 : 1);
         if (may && resident_supported(p.L) && res_need <= lds_limit && (!flips || p.ptswapever <= 255)) G.kind = 1;
         else if (may && !flips && bits_supported(p.L) && bits_lds_need(p.L, p.H) <= lds_limit + RESIDENT_SCRATCH_LDS)
             G.kind = 2;
@@ -1166,6 +1170,8 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
         E->groups.push_back(G);
     }
     E->path = c0.precision == J1J2_FP64_PLAIN ? 0 : (E->all_byte ? 1 : (E->all_resident ? 3 : 2));
+    // the launch over all sizes keeps flip counts beside each lattice
+    E->all_one = E->all_byte && (!flips || 2 * (size_t)E->max_n <= lds_limit);
     E->uniform_L = c0.L;
     E->uniform_H = c0.H;
     E->all_cubic = true;
@@ -1188,7 +1194,7 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
     if (const char *at = std::getenv("J1J2_AUTOTUNE")) E->autotune = E->autotune && std::atoi(at) != 0;
     size_t byte_lds = 0;                           // the byte-layout launches' largest lattice (and counts)
     for (const Group &G : E->groups)
-        if (G.kind == 1) byte_lds = std::max(byte_lds, (size_t)G.N * (flips ? 2 : 1));
+        if (G.kind == 1) byte_lds = std::max(byte_lds, std::min((size_t)G.N * (flips ? 2 : 1), lds_limit));
     if (byte_lds) HIPCHK(setup_resident(byte_lds));
     size_t bits_lds = 0;
     for (const Group &G : E->groups)
@@ -1271,12 +1277,12 @@ int create(const j1j2_params *groups, int32_t ngroups, int32_t device, void **ha
                 for (int S : {2, 4})
                     if (bits_slab_valid(G.p.L, G.p.H, S)) G.slab_seg[S] = add_slabs(G.conf0, G.conf0 + G.p.nconf, S);
         }
-        if (E->all_byte) E->all_seg = add(0, (int)nconf);
+        if (E->all_one) E->all_seg = add(0, (int)nconf);
         // the same in bundles for the 1024-thread launch (FP32 without the shadow
         // rule), when some size bundles and every lattice index fits an entry
         bool bundles = false;
         for (const Group &G : E->groups) bundles = bundles || res_bundle(G.p.L) > 1;
-        if (E->all_byte && !E->shadow && bundles && nlat <= BUNDLE_LAT_MASK) {
+        if (E->all_one && !E->shadow && bundles && nlat <= BUNDLE_LAT_MASK) {
             const std::vector<int32_t> seg = make_order(*E, 0, (int)nconf, 1, true);
             E->all_seg_b.ord0 = (int32_t)order.size();
             E->all_seg_b.nord = (int32_t)seg.size();

@@ -103,6 +103,17 @@ constexpr int res_bundle(int32_t L) { return L == 16 ? 8 : (L == 24 || L == 32) 
 // The resident sweeps that keep flip counts in registers add them to a byte
 // per site (flips_pend), which must not carry: PTSWAPEVER at most this.
 constexpr int32_t RC_MAX_SWAPEVER = 127;
+// Cubic L = 48 lattices with flip counts (a byte of counts beside the lattice
+// would not fit one CU's LDS) keep them in bits 3..7 of the sites' own bytes,
+// cleared at the end of each launch: PTSWAPEVER at most CB_MAX_SWAPEVER.
+constexpr int32_t CB_MAX_SWAPEVER = 31;
+constexpr bool byte_counts(int32_t L) { return L == 48; }
+// The resident kernels' flip counts need no LDS beside the lattice for these
+// (cubic FP32 lattices, PTSWAPEVER within the kernel's limit).
+inline bool fast_counts_ok(int32_t L, bool cubic, bool shadow, int32_t ptswapever)
+{
+    return cubic && !shadow && ptswapever <= (byte_counts(L) ? CB_MAX_SWAPEVER : RC_MAX_SWAPEVER);
+}
 // Bytes of LDS the resident kernels add after the lattice (reduction scratch).
 constexpr size_t RESIDENT_SCRATCH_LDS = sizeof(int64_t) * (1024 / 64) * 8;
 // Bit-layout resident sweep (kernels_bits.hip): L % 8 == 0, 48 < L <= 96, one

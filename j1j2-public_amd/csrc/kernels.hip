@@ -511,7 +511,7 @@ hipError_t launch_sweep_resident(const DeviceState &st, int32_t ord0, int32_t no
 {
     if (nord <= 0) return hipSuccess;
     const bool cubic = L > 0 ? H == L : H < 0, flips = st.flips != nullptr;
-    const ResKernel k = pick_resident(L, nt, shadow, cubic, flips && st.ptswapever <= RC_MAX_SWAPEVER);
+    const ResKernel k = pick_resident(L, nt, shadow, cubic, flips && fast_counts_ok(L, cubic, shadow, st.ptswapever));
     if (!k) return hipErrorInvalidValue;
     // with flip counts: a byte per site more, unless the kernel keeps them in registers
     const bool lds_counts = flips && k == pick_resident(L, nt, shadow, cubic, false);

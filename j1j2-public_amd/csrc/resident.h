@@ -410,17 +410,21 @@ __device__ __forceinline__ void x_views(uint32_t A, uint32_t B, uint32_t &xpe, u
     }
 }
 
-template <int WPH, int HX, bool SHADOW, int ODD, bool RC = false>
+template <int WPH, int HX, bool SHADOW, int ODD, bool RC = false, bool CB = false>
 __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int own,
                                            uint32_t m, int w, const NbPtr &P, const U4 &hi,
                                            unsigned long long &sh_prop, unsigned long long &sh_mis,
                                            uint32_t *fa = nullptr)
 {
     constexpr int HALF = 4 * WPH;
+    // CB: a site's byte is 4 b + 8 c (c its flips in this launch): the
+    // neighbours' spin bits alone enter the field
+    constexpr uint32_t NM = CB ? 0x04040404u : 0xFFFFFFFFu;
     const uint32_t S = *P.own;
-    const uint32_t YP = *P.yp, YM = *P.ym;
-    const uint32_t ZP = *P.zp, ZM = *P.zm;
-    const uint32_t XA = *P.xa, XB = *P.xb, D1A = *P.d1a, D1B = *P.d1b, D2A = *P.d2a, D2B = *P.d2b;
+    const uint32_t YP = *P.yp & NM, YM = *P.ym & NM;
+    const uint32_t ZP = *P.zp & NM, ZM = *P.zm & NM;
+    const uint32_t XA = *P.xa & NM, XB = *P.xb & NM, D1A = *P.d1a & NM, D1B = *P.d1b & NM, D2A = *P.d2a & NM,
+                   D2B = *P.d2b & NM;
     // diagonals: the two rows' words summed first (bytes 0, 4, 8: no carries),
     // so the x shift of the sum is one align
     uint32_t Dp, Dm;
@@ -507,6 +511,10 @@ __device__ __forceinline__ void update_word(uint32_t *lds32, const SweepCtx &cx,
         for (int j = 0; j < 4; ++j)
             if (((open >> j) & 1u) && decide_f32_open(ys[j], site_word(ha[j], la[j], ODD))) flip |= 4u << (8 * j);
     }
+    if constexpr (CB) {
+        *P.own = (S ^ flip) + (flip << 1);              // the spin bit toggled, c + 1 (no carry out: c < 32)
+        return;
+    }
     *P.own = S ^ flip;
     // flip counts (WRITENUMFLIPS): this thread alone touches these four sites'
     // counters in this pass, in a register (RC) or in the LDS after the lattice
@@ -540,7 +548,7 @@ __device__ __forceinline__ NbPtr odd_layer(const NbPtr &P, const uint32_t *yf, c
 // One colour pass over plane C.  HC > 0: the lattice height is HC, known at
 // compile time.  Work comes in layer pairs (2 zp, 2 zp + 1), one Philox call
 // per pair of words (m = zp PER_Z + the word's index within its layer).
-template <int WPH, int NT, int C, bool SHADOW, int HC = 0, bool RC = false>
+template <int WPH, int NT, int C, bool SHADOW, int HC = 0, bool RC = false, bool CB = false>
 __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx, const ConfConst &cc, int H,
                                             unsigned long long &sh_prop, unsigned long long &sh_mis, int tid,
                                             uint32_t *fa = nullptr)
@@ -581,11 +589,11 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
             const U4 hi = philox4x32_10_u(m, cx.ctr1, cx.sweep_lo, cx.sweep_hi, cx.key0, cx.key1);
             NbPtr Q = nb_shift(P, kk * STEP);
             if (zp == 0) Q.zm += H * PER_Z;                // periodic in z
-            update_word<WPH, HX, SHADOW, 0, RC>(lds32, cx, cc, own0 + kk * STEP, m, w, Q, hi, sh_prop, sh_mis,
+            update_word<WPH, HX, SHADOW, 0, RC, CB>(lds32, cx, cc, own0 + kk * STEP, m, w, Q, hi, sh_prop, sh_mis,
                                                 RC ? fa + 2 * kk : nullptr);
             NbPtr Q1 = nb_shift(odd_layer<PY, PER_Z>(P, yf, dfa, dfb), kk * STEP);
             if (zp == npairs - 1) Q1.zp -= H * PER_Z;
-            update_word<WPH, HX, SHADOW, 1, RC>(lds32, cx, cc, own0 + kk * STEP + PER_Z, m, w, Q1, hi, sh_prop,
+            update_word<WPH, HX, SHADOW, 1, RC, CB>(lds32, cx, cc, own0 + kk * STEP + PER_Z, m, w, Q1, hi, sh_prop,
                                                 sh_mis, RC ? fa + 2 * kk + 1 : nullptr);
         };
         if constexpr (unrolled_pairs<WPH, NT, HC>()) {
@@ -616,10 +624,10 @@ __device__ __forceinline__ void colour_pass(uint32_t *lds32, const SweepCtx &cx,
             const int t0 = 2 * zp * PER_Z + tz, t1 = t0 + PER_Z;
             const int u0 = t0 / WPH, w = t0 - u0 * WPH, u1 = u0 + HALF;
             const NbOff o0 = nb_offsets<WPH, HX>(C, nq, u0, w, unit_geo<WPH>(u0, C, H));
-            update_word<WPH, HX, SHADOW, 0>(lds32, cx, cc, C * nq + t0, (uint32_t)mm, w,
+            update_word<WPH, HX, SHADOW, 0, false, CB>(lds32, cx, cc, C * nq + t0, (uint32_t)mm, w,
                                             nb_ptrs(lds32, C * nq + t0, o0), hi, sh_prop, sh_mis);
             const NbOff o1 = nb_offsets<WPH, HX>(C, nq, u1, w, unit_geo<WPH>(u1, C, H));
-            update_word<WPH, HX, SHADOW, 1>(lds32, cx, cc, C * nq + t1, (uint32_t)mm, w,
+            update_word<WPH, HX, SHADOW, 1, false, CB>(lds32, cx, cc, C * nq + t1, (uint32_t)mm, w,
                                             nb_ptrs(lds32, C * nq + t1, o1), hi, sh_prop, sh_mis);
         }
     }
@@ -677,7 +685,7 @@ __device__ __forceinline__ void flush_reg_counts(int32_t *flips, uint8_t *pend8,
 // in their own part of the LDS).  active = false: the group repeats the work
 // of another group's lattice in its own LDS (a bundle with fewer lattices
 // than groups) and writes nothing to global memory; it takes every barrier.
-template <int WPH, int NT, bool SHADOW, int HC = 0, bool RC = false>
+template <int WPH, int NT, bool SHADOW, int HC = 0, bool RC = false, bool CB = false>
 __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDesc &d, int lat,
                                               uint32_t *lds32, int64_t (*red)[OBS_N], int tid = threadIdx.x,
                                               bool active = true)
@@ -768,7 +776,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     // flip counts of this launch, one byte per site right after the lattice
     // (at most PTSWAPEVER <= 255 per site), added to the slot's counters at the end
     const bool counting = st.flips != nullptr;
-    if (counting && !RC)
+    if (counting && !RC && !CB)
         for (int i = tid; i < N / 4; i += NT) lds32[N / 4 + i] = 0u;
     // RC: the counts of the words a thread updates stay in its registers
     uint32_t fa[4][RC ? 2 * pair_steps<WPH, NT, HC>() : 1] = {};
@@ -789,7 +797,7 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
     cx.beta = SHADOW ? st.beta[d.slot_base + slot] : 0.0;
     cx.key0 = st.seed;
     cx.key1 = (uint32_t)cd.iconf;
-    cx.cnt_off = counting && !RC ? N / 4 : 0;
+    cx.cnt_off = counting && !RC && !CB ? N / 4 : 0;
     const uint64_t S0 = clock_cycle(st) * (uint64_t)st.ptswapever;
     unsigned long long sh_prop = 0, sh_mis = 0;
     __syncthreads();
@@ -802,15 +810,32 @@ __device__ __forceinline__ void resident_body(const DeviceState &st, const LatDe
         for (int c = 0; c < 4; ++c) {
             cx.ctr1 = (uint32_t)c | ((uint32_t)d.rep << 2);
             switch (c) {
-                case 0: colour_pass<WPH, NT, 0, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[0]); break;
-                case 1: colour_pass<WPH, NT, 1, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[1]); break;
-                case 2: colour_pass<WPH, NT, 2, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[2]); break;
-                default: colour_pass<WPH, NT, 3, SHADOW, HC, RC>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[3]); break;
+                case 0: colour_pass<WPH, NT, 0, SHADOW, HC, RC, CB>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[0]); break;
+                case 1: colour_pass<WPH, NT, 1, SHADOW, HC, RC, CB>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[1]); break;
+                case 2: colour_pass<WPH, NT, 2, SHADOW, HC, RC, CB>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[2]); break;
+                default: colour_pass<WPH, NT, 3, SHADOW, HC, RC, CB>(lds32, cx, cc, H, sh_prop, sh_mis, tid, fa[3]); break;
             }
             __syncthreads();
         }
     }
-    if constexpr (RC) {
+    if constexpr (CB) {
+        // confnumflips(slot, site) += the counts in bits 3..7 of the sites'
+        // bytes, which are cleared before anything else reads the lattice
+        int4 *fl = reinterpret_cast<int4 *>(st.flips + cd.flip_off + (int64_t)slot * N);
+        for (int i = tid; i < N / 4; i += NT) {
+            const uint32_t w = lds32[i], n4 = (w >> 3) & 0x1F1F1F1Fu;
+            if (n4 && active && counting) {
+                int4 g = fl[i];
+                g.x += (int)(n4 & 0xFFu);
+                g.y += (int)((n4 >> 8) & 0xFFu);
+                g.z += (int)((n4 >> 16) & 0xFFu);
+                g.w += (int)(n4 >> 24);
+                fl[i] = g;
+            }
+            lds32[i] = w & 0x04040404u;
+        }
+        __syncthreads();
+    } else if constexpr (RC) {
         if (counting && active)
             flush_reg_counts<WPH, NT, HC>(st.flips + cd.flip_off + (int64_t)slot * N,
                                           st.flips_pend + cd.flip_off + (int64_t)slot * N, fa, tid);
@@ -927,8 +952,9 @@ constexpr bool reg_counts()
 }
 
 // One geometry.  HC > 0: every lattice of the launch has height HC (cubic
-// lattices, H = L).  RC: flip counts in registers (reg_counts), none in the LDS.
-template <int WPH, int NT, bool SHADOW, int HC = 0, bool RC = false>
+// lattices, H = L).  RC: flip counts in registers (reg_counts), none in the
+// LDS.  CB: flip counts in the sites' own bytes (byte_counts).
+template <int WPH, int NT, bool SHADOW, int HC = 0, bool RC = false, bool CB = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SHADOW ? 1 : res_waves<WPH, NT, HC>(), 8)))
 void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
 {
@@ -938,8 +964,8 @@ void k_sweep_resident(DeviceState st, int32_t ord0, int32_t nord)
     const int lat = st.order[ord0 + blockIdx.x];
     if (lat < 0) return;
     const LatDesc d = st.lats[lat];
-    const int lds_words = (st.flips && !RC ? 2 : 1) * (d.N / 4);     // lattice (and flip counts)
-    resident_body<WPH, NT, SHADOW, HC, RC>(st, d, lat, lds32,
+    const int lds_words = (st.flips && !RC && !CB ? 2 : 1) * (d.N / 4);     // lattice (and flip counts)
+    resident_body<WPH, NT, SHADOW, HC, RC, CB>(st, d, lat, lds32,
                                            reinterpret_cast<int64_t (*)[OBS_N]>(lds32 + lds_words));
 }
 
@@ -1011,6 +1037,8 @@ ResKernel cubic_kernel(bool flips)
 {
     if constexpr (reg_counts<WPH, NT, 8 * WPH>())
         if (flips) return k_sweep_resident<WPH, NT, false, 8 * WPH, true>;
+    if constexpr (byte_counts(8 * WPH))
+        if (flips) return k_sweep_resident<WPH, NT, false, 8 * WPH, false, true>;
     return k_sweep_resident<WPH, NT, false, 8 * WPH>;
 }
 

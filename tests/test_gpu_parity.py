@@ -263,15 +263,31 @@ def test_flip_counts_l32_match_oracle(monkeypatch, nt, swapever, neq, nmess):
     assert n == 2 and eng.sweep_path()[0] == 1 and f"L32 nt{nt}" in eng.launch_plan(), eng.launch_plan()
 
 
+@pytest.mark.parametrize("nt,swapever,path", [(256, 5, 1), (576, 31, 1), (1024, 5, 1), (1024, 32, 2)])
+def test_flip_counts_l48_match_oracle(monkeypatch, nt, swapever, path):
+    """Flip counts of cubic L = 48 lattices (lattice and a byte of counts
+    beside it would not fit one CU's LDS): in bits 3..7 of the sites' own
+    bytes on the LDS-resident sweep for PTSWAPEVER <= 31, cleared at the end
+    of each launch; the global-memory sweep above that."""
+    say(f"case WRITENUMFLIPS L=48 nt {nt} PTSWAPEVER {swapever}")
+    monkeypatch.setenv("J1J2_RES_THREADS", str(nt))
+    g = J.make_params(48, 48, nreplica=2, nconf=2, iconf0=4, seed=SEED, ptmeasindex=2, ptswapever=swapever,
+                      flags=J.FLAG_WRITENUMFLIPS | LOCNEM)
+    eng, n = run_and_compare([g], ["hot", "str"], 3, 4, flips=True)
+    assert n == 2 and eng.sweep_path()[0] == path, eng.launch_plan()
+
+
 def test_flip_counts_uneven_batch_match_oracle():
-    """Flip counts over an uneven batch: L = 16, 24 and 40 on the LDS-resident
-    sweep (lattice and counts fit one CU's LDS), L = 48 on the global-memory
-    sweep in the same run, 20 cycles (counts accumulate over launches)."""
+    """Flip counts over an uneven batch: L = 16, 24 and 40 with a byte of
+    counts beside the lattice in LDS, cubic L = 48 with the counts in the
+    sites' own bytes, per-size launches (one launch over all sizes would need
+    the counts' LDS for L = 48 too), 20 cycles (counts accumulate over
+    launches)."""
     say("case WRITENUMFLIPS uneven batch")
     groups = [J.make_params(L, L, nreplica=R, nconf=1, iconf0=1 + k, seed=SEED, flags=J.FLAG_WRITENUMFLIPS)
               for k, (L, R) in enumerate([(16, 3), (24, 2), (40, 2), (48, 2)])]
-    eng, _ = run_and_compare(groups, ["hot", "str", "hot", "str"], 8, 12, flips=True, ora_configs=[0, 1, 2])
-    assert eng.sweep_path()[0] == 2
+    eng, _ = run_and_compare(groups, ["hot", "str", "hot", "str"], 8, 12, flips=True)
+    assert eng.sweep_path()[0] == 1 and "L0" not in eng.launch_plan(), eng.launch_plan()
 
 
 @pytest.mark.parametrize("precision,flags,R,ptswapever,ptmeasindex,bins", [
